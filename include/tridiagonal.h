@@ -1,0 +1,175 @@
+/*
+ * tridiagonal.h -- C-ABI of the MI355X batched tridiagonal solver library
+ * (libtridiag_amd.so).  Plain pointers and sizes only.
+ *
+ * Part 1 keeps the entry-point surface of the reference's
+ * Tridiagonal_CUDA/tridiagonal.h:8-57 so that its test driver (test.cpp,
+ * helpers.h solver table) and run_all.sh sweep link against this library
+ * unchanged.  Semantics of every Part-1 symbol, as in the reference:
+ *   - a_gbl, b_gbl, c_gbl, d_gbl are device pointers (hipMalloc), n_eqt rows
+ *     per system, n_batch systems, layout 0: row i of system s at [s*n_eqt + i];
+ *   - the solve is in place: x is returned in d_gbl; c_gbl may be clobbered;
+ *   - a_gbl[s*n_eqt] (a_0) and c_gbl[s*n_eqt + n_eqt-1] (c_{n-1}) are ignored;
+ *   - the call is synchronous (ends with a device synchronisation), runs on
+ *     the null stream of the current device and returns nothing.  Unlike the
+ *     reference, bad arguments or launch failures do not exit the process:
+ *     they are printed to stderr and kept in tridiag_last_error().
+ *   - the reference's size preconditions (n%64 for WM, n%32 for PCR, n%R for
+ *     CR_reg R, batch >= 2) do not apply: every n >= 1 and batch >= 0 work.
+ *
+ * Part 2 adds status-returning, stream-ordered entry points with an explicit
+ * layout (0 contiguous, 1 packed a|b|c|d per system, 2 interleaved
+ * [i*batch + s]), ragged batches and the adjoint (backward) solve.
+ */
+#ifndef TRIDIAGONAL_AMD_H_INCLUDED
+#define TRIDIAGONAL_AMD_H_INCLUDED 1
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================================================================== Part 1
+ * Reference-compatible symbols.  Each line cites the declaration it replaces
+ * in Tridiagonal_CUDA/tridiagonal.h.
+ */
+
+/* tridiagonal.h:8-12 -- vendor library comparator (cuSPARSE gtsv2StridedBatch
+ * in the reference; hipSPARSE gtsv2StridedBatch here).  init allocates the
+ * library workspace and returns 0 on success. */
+int cuSparse_init_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+int cuSparse_init_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+void cuSparse_final(void);
+void cuSparse_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+void cuSparse_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+
+/* tridiagonal.h:15-18 -- BPLG entry points (FP32, packed layout 1: one buffer,
+ * system s at [4*n*s, 4*n*(s+1)) holding a|b|c|d; only a_gbl is read, x is
+ * returned in the d slot).  Solved by this library's own kernels; the BPLG
+ * restriction to power-of-two n <= 2048 does not apply. */
+void BPLG_WM(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+void BPLG_LF(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+void BPLG_CR(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+void BPLG_PCR(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+
+/* tridiagonal.h:21-22 -- naive Thomas, one lane per system. */
+void Thomas_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+void Thomas_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+
+/* tridiagonal.h:25-29 -- cyclic reduction.  init/final manage the scratch
+ * buffer (4*n*batch words) used by the global-memory levels; init returns 0 on
+ * success. */
+int CR_init_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+int CR_init_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+void CR_final(void);
+void CR_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+void CR_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+
+/* tridiagonal.h:31-34 -- LDS-staged ("shmem") PCR-pThomas / WM-pGE, FP32. */
+void WM5_shmem_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+void WM4_shmem_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+void PCR5_shmem_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+void PCR4_shmem_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+
+/* tridiagonal.h:36-39 -- register-resident ("reg") variants, FP32. */
+void WM5_reg_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+void WM4_reg_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+void PCR5_reg_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+void PCR4_reg_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+
+/* tridiagonal.h:41-44 -- LDS-staged variants, FP64. */
+void WM5_shmem_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+void WM4_shmem_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+void PCR5_shmem_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+void PCR4_shmem_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+
+/* tridiagonal.h:46-49 -- register-resident variants, FP64. */
+void WM5_reg_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+void WM4_reg_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+void PCR5_reg_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+void PCR4_reg_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+
+/* tridiagonal.h:52-57 -- register-packed CR, R = 4, 8, 16 rows per lane. */
+void CR_reg4_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+void CR_reg8_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+void CR_reg16_double(double *a_gbl, double *b_gbl, double *c_gbl, double *d_gbl, int n_eqt, int n_batch);
+void CR_reg4_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+void CR_reg8_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+void CR_reg16_single(float *a_gbl, float *b_gbl, float *c_gbl, float *d_gbl, int n_eqt, int n_batch);
+
+/* ===================================================================== Part 2
+ * Status-returning API (no counterpart in the reference; its callers are this
+ * library's test driver, the Python binding and the benchmark).
+ * Return codes: */
+#define TRIDIAG_OK 0
+#define TRIDIAG_EINVAL 1      /* bad argument (null pointer, n < 1, bad id)   */
+#define TRIDIAG_EUNSUPPORTED 2/* combination not provided by this build       */
+#define TRIDIAG_EHIP 3        /* HIP runtime error (launch, memory)           */
+
+#define TRIDIAG_SINGLE 0      /* precision ids, as the reference CLI (test.cpp:141-149) */
+#define TRIDIAG_DOUBLE 1
+
+#define TRIDIAG_LAYOUT_CONTIGUOUS 0
+#define TRIDIAG_LAYOUT_PACKED 1
+#define TRIDIAG_LAYOUT_INTERLEAVED 2
+
+/* Algorithm ids = indices of the reference solver table
+ * (Tridiagonal_CUDA/helpers.h:23-66): 0 vendor gtsv, 1 Thomas, 2 CR,
+ * 3 shmem PCR-pThomas, 4 shmem PCR-half-pThomas, 5 reg PCR-pThomas,
+ * 6 reg PCR-half-pThomas, 7 shmem WM-pGE, 8 shmem WM-half-pGE, 9 reg WM-pGE,
+ * 10 reg WM-half-pGE, 11 reg-4 CR, 12 reg-8 CR, 13 reg-16 CR; FP32 only:
+ * 14 BPLG WM, 15 BPLG LF, 16 BPLG CR, 17 BPLG PCR (packed layout).
+ * TRIDIAG_ALGO_AUTO picks the fastest kernel for (precision, layout, n). */
+#define TRIDIAG_ALGO_AUTO (-1)
+
+int tridiag_algo_count(int precision);
+const char *tridiag_algo_name(int precision, int algo);
+/* Native layout of an algorithm in the reference table (helpers.h:17). */
+int tridiag_algo_layout(int precision, int algo);
+
+/* Solve n_batch systems of n_eqt rows in place (x -> d), enqueued on
+ * `stream` (a hipStream_t; NULL = null stream).  Asynchronous. */
+int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch,
+                  void *a, void *b, void *c, void *d, void *stream);
+
+/* Ragged batch (layout 0 per system, systems concatenated): system s holds rows
+ * [offsets[s], offsets[s+1]) of a, b, c, d.  offsets is a device array of
+ * n_sys+1 int64.  One persistent launch; x -> d.  plan_bytes of device memory
+ * at `work` (from tridiag_ragged_work_size) hold the schedule. */
+int64_t tridiag_ragged_work_size(int64_t n_sys);
+int tridiag_solve_ragged(int precision, int64_t n_sys, const int64_t *offsets_host,
+                         const int64_t *offsets_dev, void *a, void *b, void *c, void *d,
+                         void *work, void *stream);
+
+/* Adjoint of the solve.  Given a, b, c, the forward solution x and the upstream
+ * gradient g = dL/dx (all layout `layout`, layouts 0 and 2), writes
+ *   gd = lambda = A^{-T} g,  ga_i = -lambda_i x_{i-1},  gb_i = -lambda_i x_i,
+ *   gc_i = -lambda_i x_{i+1}  (ga_0 = gc_{n-1} = 0).
+ * Outputs may not alias inputs.  Asynchronous on `stream`. */
+int tridiag_adjoint(int precision, int layout, int64_t n_eqt, int64_t n_batch, const void *a,
+                    const void *b, const void *c, const void *x, const void *g, void *ga,
+                    void *gb, void *gc, void *gd, void *stream);
+
+/* Fused forward + adjoint for a known upstream gradient g (e.g. a linear loss):
+ * x is written to `x`, the four gradients as in tridiag_adjoint.  Reads
+ * a, b, c, d, g once.  Asynchronous on `stream`. */
+int tridiag_solve_fwd_bwd(int precision, int layout, int64_t n_eqt, int64_t n_batch,
+                          const void *a, const void *b, const void *c, const void *d,
+                          const void *g, void *x, void *ga, void *gb, void *gc, void *gd,
+                          void *stream);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *tridiag_last_error(void);
+
+/* Kernel configuration the dispatcher would use (for reports and tuning):
+ * fills rows-per-lane M, waves per workgroup NW, systems per workgroup S.
+ * Returns TRIDIAG_OK or TRIDIAG_EUNSUPPORTED. */
+int tridiag_plan(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch, int *M,
+                 int *NW, int *S);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,95 @@
+"""ctypes binding of libtridiag_amd.so (the C-ABI in include/tridiagonal.h).
+
+The HIP library is the only compute path: if it is missing this module raises
+instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libtridiag_amd.so")
+
+SINGLE, DOUBLE = 0, 1
+CONTIGUOUS, PACKED, INTERLEAVED = 0, 1, 2
+ALGO_AUTO = -1
+OK, EINVAL, EUNSUPPORTED, EHIP = 0, 1, 2, 3
+
+# Reference-compatible void symbols (Tridiagonal_CUDA/tridiagonal.h:8-57).
+REFERENCE_SYMBOLS = [
+    "cuSparse_init_single", "cuSparse_init_double", "cuSparse_final", "cuSparse_single",
+    "cuSparse_double", "BPLG_WM", "BPLG_LF", "BPLG_CR", "BPLG_PCR", "Thomas_single",
+    "Thomas_double", "CR_init_single", "CR_init_double", "CR_final", "CR_single", "CR_double",
+    "WM5_shmem_single", "WM4_shmem_single", "PCR5_shmem_single", "PCR4_shmem_single",
+    "WM5_reg_single", "WM4_reg_single", "PCR5_reg_single", "PCR4_reg_single",
+    "WM5_shmem_double", "WM4_shmem_double", "PCR5_shmem_double", "PCR4_shmem_double",
+    "WM5_reg_double", "WM4_reg_double", "PCR5_reg_double", "PCR4_reg_double",
+    "CR_reg4_double", "CR_reg8_double", "CR_reg16_double", "CR_reg4_single", "CR_reg8_single",
+    "CR_reg16_single",
+]
+EXTENDED_SYMBOLS = [
+    "tridiag_algo_count", "tridiag_algo_name", "tridiag_algo_layout", "tridiag_solve",
+    "tridiag_ragged_work_size", "tridiag_solve_ragged", "tridiag_adjoint",
+    "tridiag_solve_fwd_bwd", "tridiag_last_error", "tridiag_plan",
+]
+
+
+class TridiagError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("tridiag error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libtridiag_amd.so not built (run python -m tridiagonalsolver_amd.build)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, ci = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+    L.tridiag_algo_count.argtypes = [ci]
+    L.tridiag_algo_count.restype = ci
+    L.tridiag_algo_name.argtypes = [ci, ci]
+    L.tridiag_algo_name.restype = ctypes.c_char_p
+    L.tridiag_algo_layout.argtypes = [ci, ci]
+    L.tridiag_algo_layout.restype = ci
+    L.tridiag_solve.argtypes = [ci, ci, ci, i64, i64, vp, vp, vp, vp, vp]
+    L.tridiag_solve.restype = ci
+    L.tridiag_ragged_work_size.argtypes = [i64]
+    L.tridiag_ragged_work_size.restype = i64
+    L.tridiag_solve_ragged.argtypes = [ci, i64, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.tridiag_solve_ragged.restype = ci
+    L.tridiag_adjoint.argtypes = [ci, ci, i64, i64] + [vp] * 10
+    L.tridiag_adjoint.restype = ci
+    L.tridiag_solve_fwd_bwd.argtypes = [ci, ci, i64, i64] + [vp] * 11
+    L.tridiag_solve_fwd_bwd.restype = ci
+    L.tridiag_last_error.argtypes = []
+    L.tridiag_last_error.restype = ctypes.c_char_p
+    L.tridiag_plan.argtypes = [ci, ci, ci, i64, i64, ctypes.POINTER(ci), ctypes.POINTER(ci),
+                               ctypes.POINTER(ci)]
+    L.tridiag_plan.restype = ci
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != OK:
+        raise TridiagError(rc, lib().tridiag_last_error().decode())
+    return rc
+
+
+def algo_names(precision):
+    L = lib()
+    return [L.tridiag_algo_name(precision, i).decode()
+            for i in range(L.tridiag_algo_count(precision))]
+
+
+def plan(precision, algo, layout, n, batch):
+    M, NW, S = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(lib().tridiag_plan(precision, algo, layout, n, batch, ctypes.byref(M), ctypes.byref(NW),
+                             ctypes.byref(S)))
+    return dict(M=M.value, NW=NW.value, S=S.value)

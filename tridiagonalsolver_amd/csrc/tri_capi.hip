@@ -1,0 +1,371 @@
+// C-ABI of libtridiag_amd.so: the reference-compatible symbols of
+// Tridiagonal_CUDA/tridiagonal.h:8-57 plus the status-returning tridiag_* API.
+#include <hip/hip_runtime.h>
+#include <hipsparse/hipsparse.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "../../include/tridiagonal.h"
+#include "tri_host.hpp"
+
+namespace tri {
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+static int hip_fail(hipError_t e, const char *where) {
+    return fail(TRIDIAG_EHIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------- algorithm table
+// Names and order of the reference solver table (Tridiagonal_CUDA/helpers.h:23-66).
+enum Algo {
+    kVendor = 0, kThomas = 1, kCR = 2, kPCR5Shmem = 3, kPCR4Shmem = 4, kPCR5Reg = 5, kPCR4Reg = 6,
+    kWM5Shmem = 7, kWM4Shmem = 8, kWM5Reg = 9, kWM4Reg = 10, kCRReg4 = 11, kCRReg8 = 12,
+    kCRReg16 = 13, kBPLGWM = 14, kBPLGLF = 15, kBPLGCR = 16, kBPLGPCR = 17
+};
+
+static const char *const kNamesSingle[] = {
+    "hipSPARSE sgtsv", "Thomas", "CR", "shmem PCR-pThomas", "shmem PCR-half-pThomas",
+    "reg PCR-pThomas", "reg PCR-half-pThomas", "shmem WM-pGE", "shmem WM-half-pGE",
+    "reg WM-pGE", "reg WM-half-pGE", "reg-4 CR", "reg-8 CR", "reg-16 CR",
+    "BPLG WM", "BPLG LF", "BPLG CR", "BPLG PCR"};
+static const char *const kNamesDouble[] = {
+    "hipSPARSE dgtsv", "Thomas", "CR", "shmem PCR-pThomas", "shmem PCR-half-pThomas",
+    "reg PCR-pThomas", "reg PCR-half-pThomas", "shmem WM-pGE", "shmem WM-half-pGE",
+    "reg WM-pGE", "reg WM-half-pGE", "reg-4 CR", "reg-8 CR", "reg-16 CR"};
+
+static int algo_count(int prec) { return prec == TRIDIAG_SINGLE ? 18 : 14; }
+
+// ---------------------------------------------------------------- planning
+template <typename T> static int pref_M() { return sizeof(T) == 4 ? 32 : 16; }
+template <typename T> static int pref_S_il() { return 32 / (int)sizeof(T); }
+
+// Rows per lane fixed by the algorithm (register-packed CR: R rows per lane).
+static int forced_M(int algo) {
+    switch (algo) {
+    case kCRReg4: return 4;
+    case kCRReg8: return 8;
+    case kCRReg16: return 16;
+    default: return 0;
+    }
+}
+
+template <typename T>
+static const TileEntry<T> *choose_tile(int algo, int layout, int64_t n, bool vec_ok) {
+    TileTable<T> tab = tile_table<T>();
+    const bool il = layout == kInterleaved;
+    const int fM = forced_M(algo);
+    const TileEntry<T> *best = nullptr;
+    long long best_key[5] = {0, 0, 0, 0, 0};
+    for (int i = 0; i < tab.count; ++i) {
+        const TileEntry<T> &e = tab.entries[i];
+        if (e.IL != il || e.cap < n) continue;
+        if (e.VEC && !vec_ok) continue;
+        if (fM && e.M != fM) continue;
+        const int nws = il ? e.NW : e.NW / e.S;
+        long long key[5] = {
+            il ? (e.S == pref_S_il<T>() ? 0 : 1) : 0,  // 32-byte row groups in IL
+            (long long)e.cap,                          // least padding
+            il ? 0 : (long long)nws,                   // fewest waves per system
+            (long long)std::abs(e.M - pref_M<T>()),    // moderate register use
+            e.VEC ? 0 : 1};
+        if (!best || std::lexicographical_compare(key, key + 5, best_key, best_key + 5)) {
+            best = &e;
+            std::copy(key, key + 5, best_key);
+        }
+    }
+    return best;
+}
+
+template <typename T> static bool aligned16(const void *p) {
+    return (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+}
+
+template <typename T>
+static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b, T *c, T *d,
+                      hipStream_t st) {
+    if (n < 1) return fail(TRIDIAG_EINVAL, "n_eqt must be >= 1");
+    if (batch < 0) return fail(TRIDIAG_EINVAL, "n_batch must be >= 0");
+    if (layout < 0 || layout > 2) return fail(TRIDIAG_EINVAL, "layout must be 0, 1 or 2");
+    if (!a || (layout != kPacked && (!b || !c || !d)))
+        return fail(TRIDIAG_EINVAL, "null coefficient pointer");
+    if (batch == 0) return TRIDIAG_OK;
+    int64_t sstride = n, rstride = 1;
+    if (layout == kPacked) {
+        b = a + n; c = a + 2 * n; d = a + 3 * n;
+        sstride = 4 * n;
+    } else if (layout == kInterleaved) {
+        sstride = 1;
+        rstride = batch;
+    }
+    if (algo == kThomas) {
+        hipError_t e = (sizeof(T) == 4)
+            ? launch_thomas_f32((const float *)a, (const float *)b, (float *)c, (float *)d, n,
+                                batch, sstride, rstride, st)
+            : launch_thomas_f64((const double *)a, (const double *)b, (double *)c, (double *)d, n,
+                                batch, sstride, rstride, st);
+        return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "Thomas launch");
+    }
+    const int V = 16 / (int)sizeof(T);
+    const bool vec_ok = layout != kInterleaved && (sstride % V) == 0 && aligned16<T>(a) &&
+                        aligned16<T>(b) && aligned16<T>(c) && aligned16<T>(d);
+    const TileEntry<T> *e = choose_tile<T>(algo, layout, n, vec_ok);
+    if (!e) {
+        return fail(TRIDIAG_EUNSUPPORTED,
+                    "n_eqt=" + std::to_string(n) + " exceeds the on-chip capacity of algorithm " +
+                        std::to_string(algo) + " in layout " + std::to_string(layout));
+    }
+    TileParams<T> p{a, b, c, d, n, batch, sstride, rstride};
+    hipError_t err = e->launch(p, st);
+    return err == hipSuccess ? TRIDIAG_OK : hip_fail(err, "tile launch");
+}
+
+// ---------------------------------------------------------------- vendor comparator
+static hipsparseHandle_t g_sparse = nullptr;
+static void *g_sparse_buf = nullptr;
+static size_t g_sparse_bytes = 0;
+static void *g_cr_buf = nullptr;
+
+template <typename T> static int sparse_init(T *a, T *b, T *c, T *d, int n, int batch) {
+    if (!g_sparse) {
+        if (hipsparseCreate(&g_sparse) != HIPSPARSE_STATUS_SUCCESS)
+            return fail(TRIDIAG_EHIP, "hipsparseCreate failed");
+    }
+    size_t bytes = 0;
+    hipsparseStatus_t s;
+    if (sizeof(T) == 4)
+        s = hipsparseSgtsv2StridedBatch_bufferSizeExt(g_sparse, n, (float *)a, (float *)b,
+                                                      (float *)c, (float *)d, batch, n, &bytes);
+    else
+        s = hipsparseDgtsv2StridedBatch_bufferSizeExt(g_sparse, n, (double *)a, (double *)b,
+                                                      (double *)c, (double *)d, batch, n, &bytes);
+    if (s != HIPSPARSE_STATUS_SUCCESS) return fail(TRIDIAG_EHIP, "gtsv2StridedBatch_bufferSizeExt");
+    if (!g_sparse_buf || bytes > g_sparse_bytes) {
+        if (g_sparse_buf) (void)hipFree(g_sparse_buf);
+        g_sparse_buf = nullptr;
+        g_sparse_bytes = 0;
+        hipError_t e = hipMalloc(&g_sparse_buf, bytes ? bytes : 16);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(gtsv buffer)");
+        g_sparse_bytes = bytes;
+    }
+    return TRIDIAG_OK;
+}
+
+template <typename T> static int sparse_solve(T *a, T *b, T *c, T *d, int n, int batch) {
+    if (!g_sparse || !g_sparse_buf) return fail(TRIDIAG_EINVAL, "cuSparse_init was not called");
+    hipsparseStatus_t s;
+    if (sizeof(T) == 4)
+        s = hipsparseSgtsv2StridedBatch(g_sparse, n, (float *)a, (float *)b, (float *)c,
+                                        (float *)d, batch, n, g_sparse_buf);
+    else
+        s = hipsparseDgtsv2StridedBatch(g_sparse, n, (double *)a, (double *)b, (double *)c,
+                                        (double *)d, batch, n, g_sparse_buf);
+    if (s != HIPSPARSE_STATUS_SUCCESS) return fail(TRIDIAG_EHIP, "gtsv2StridedBatch failed");
+    return TRIDIAG_OK;
+}
+
+// Synchronous reference-style call: solve on the null stream, then wait.
+template <typename T> static void ref_call(int algo, int layout, T *a, T *b, T *c, T *d, int n,
+                                           int batch) {
+    int rc = solve_impl<T>(algo, layout, n, batch, a, b, c, d, nullptr);
+    if (rc == TRIDIAG_OK) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) rc = hip_fail(e, "hipDeviceSynchronize");
+    }
+    if (rc != TRIDIAG_OK) {
+        fprintf(stderr, "tridiag_amd: %s (algo %d) failed: %s\n",
+                sizeof(T) == 4 ? kNamesSingle[algo] : kNamesDouble[algo], algo,
+                g_last_error.c_str());
+    }
+}
+
+}  // namespace tri
+
+using namespace tri;
+
+// ======================================================================= Part 2
+extern "C" {
+
+int tridiag_algo_count(int precision) {
+    if (precision != TRIDIAG_SINGLE && precision != TRIDIAG_DOUBLE) return 0;
+    return algo_count(precision);
+}
+
+const char *tridiag_algo_name(int precision, int algo) {
+    if (algo < 0 || algo >= tridiag_algo_count(precision)) return nullptr;
+    return precision == TRIDIAG_SINGLE ? kNamesSingle[algo] : kNamesDouble[algo];
+}
+
+int tridiag_algo_layout(int precision, int algo) {
+    if (algo < 0 || algo >= tridiag_algo_count(precision)) return -1;
+    return algo >= kBPLGWM ? TRIDIAG_LAYOUT_PACKED : TRIDIAG_LAYOUT_CONTIGUOUS;
+}
+
+int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch, void *a,
+                  void *b, void *c, void *d, void *stream) {
+    g_last_error.clear();
+    if (precision != TRIDIAG_SINGLE && precision != TRIDIAG_DOUBLE)
+        return fail(TRIDIAG_EINVAL, "precision must be 0 (single) or 1 (double)");
+    if (algo == TRIDIAG_ALGO_AUTO) algo = kWM5Reg;
+    if (algo < 0 || algo >= algo_count(precision))
+        return fail(TRIDIAG_EINVAL, "unknown algorithm id " + std::to_string(algo));
+    if (algo == kVendor) {
+        if (layout != kContiguous) return fail(TRIDIAG_EUNSUPPORTED, "gtsv needs layout 0");
+        if (n_eqt > 0x7fffffff || n_batch > 0x7fffffff) return fail(TRIDIAG_EINVAL, "size");
+        int rc = (precision == TRIDIAG_SINGLE)
+                     ? sparse_init<float>((float *)a, (float *)b, (float *)c, (float *)d,
+                                          (int)n_eqt, (int)n_batch)
+                     : sparse_init<double>((double *)a, (double *)b, (double *)c, (double *)d,
+                                           (int)n_eqt, (int)n_batch);
+        if (rc) return rc;
+        if (hipsparseSetStream(g_sparse, (hipStream_t)stream) != HIPSPARSE_STATUS_SUCCESS)
+            return fail(TRIDIAG_EHIP, "hipsparseSetStream");
+        return (precision == TRIDIAG_SINGLE)
+                   ? sparse_solve<float>((float *)a, (float *)b, (float *)c, (float *)d,
+                                         (int)n_eqt, (int)n_batch)
+                   : sparse_solve<double>((double *)a, (double *)b, (double *)c, (double *)d,
+                                          (int)n_eqt, (int)n_batch);
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == TRIDIAG_SINGLE)
+        return solve_impl<float>(algo, layout, n_eqt, n_batch, (float *)a, (float *)b,
+                                 (float *)c, (float *)d, st);
+    return solve_impl<double>(algo, layout, n_eqt, n_batch, (double *)a, (double *)b,
+                              (double *)c, (double *)d, st);
+}
+
+int tridiag_plan(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch, int *M,
+                 int *NW, int *S) {
+    (void)n_batch;
+    if (algo == TRIDIAG_ALGO_AUTO) algo = kWM5Reg;
+    if (algo <= kThomas) return fail(TRIDIAG_EUNSUPPORTED, "no tile plan for this algorithm");
+    int m = 0, nw = 0, s = 0;
+    bool found = false;
+    if (precision == TRIDIAG_SINGLE) {
+        const TileEntry<float> *e = choose_tile<float>(algo, layout, n_eqt, layout != 2);
+        if (e) { m = e->M; nw = e->NW; s = e->S; found = true; }
+    } else {
+        const TileEntry<double> *e = choose_tile<double>(algo, layout, n_eqt, layout != 2);
+        if (e) { m = e->M; nw = e->NW; s = e->S; found = true; }
+    }
+    if (!found) return fail(TRIDIAG_EUNSUPPORTED, "n_eqt exceeds on-chip capacity");
+    if (M) *M = m;
+    if (NW) *NW = nw;
+    if (S) *S = s;
+    return TRIDIAG_OK;
+}
+
+int64_t tridiag_ragged_work_size(int64_t n_sys) { return n_sys < 0 ? 0 : 64 * (n_sys + 1); }
+
+int tridiag_solve_ragged(int precision, int64_t n_sys, const int64_t *offsets_host,
+                         const int64_t *offsets_dev, void *a, void *b, void *c, void *d,
+                         void *work, void *stream) {
+    (void)precision; (void)n_sys; (void)offsets_host; (void)offsets_dev; (void)a; (void)b;
+    (void)c; (void)d; (void)work; (void)stream;
+    return fail(TRIDIAG_EUNSUPPORTED, "ragged solve not built yet");
+}
+
+int tridiag_adjoint(int precision, int layout, int64_t n_eqt, int64_t n_batch, const void *a,
+                    const void *b, const void *c, const void *x, const void *g, void *ga,
+                    void *gb, void *gc, void *gd, void *stream) {
+    (void)precision; (void)layout; (void)n_eqt; (void)n_batch; (void)a; (void)b; (void)c;
+    (void)x; (void)g; (void)ga; (void)gb; (void)gc; (void)gd; (void)stream;
+    return fail(TRIDIAG_EUNSUPPORTED, "adjoint not built yet");
+}
+
+int tridiag_solve_fwd_bwd(int precision, int layout, int64_t n_eqt, int64_t n_batch,
+                          const void *a, const void *b, const void *c, const void *d,
+                          const void *g, void *x, void *ga, void *gb, void *gc, void *gd,
+                          void *stream) {
+    (void)precision; (void)layout; (void)n_eqt; (void)n_batch; (void)a; (void)b; (void)c;
+    (void)d; (void)g; (void)x; (void)ga; (void)gb; (void)gc; (void)gd; (void)stream;
+    return fail(TRIDIAG_EUNSUPPORTED, "fused forward+adjoint not built yet");
+}
+
+const char *tridiag_last_error(void) { return g_last_error.c_str(); }
+
+// ======================================================================= Part 1
+#define REF_SYMBOL(NAME, T, ALGO, LAYOUT)                                                  \
+    void NAME(T *a_gbl, T *b_gbl, T *c_gbl, T *d_gbl, int n_eqt, int n_batch) {            \
+        ref_call<T>(ALGO, LAYOUT, a_gbl, b_gbl, c_gbl, d_gbl, n_eqt, n_batch);             \
+    }
+
+REF_SYMBOL(Thomas_single, float, kThomas, kContiguous)
+REF_SYMBOL(Thomas_double, double, kThomas, kContiguous)
+REF_SYMBOL(CR_single, float, kCR, kContiguous)
+REF_SYMBOL(CR_double, double, kCR, kContiguous)
+REF_SYMBOL(WM5_shmem_single, float, kWM5Shmem, kContiguous)
+REF_SYMBOL(WM4_shmem_single, float, kWM4Shmem, kContiguous)
+REF_SYMBOL(PCR5_shmem_single, float, kPCR5Shmem, kContiguous)
+REF_SYMBOL(PCR4_shmem_single, float, kPCR4Shmem, kContiguous)
+REF_SYMBOL(WM5_reg_single, float, kWM5Reg, kContiguous)
+REF_SYMBOL(WM4_reg_single, float, kWM4Reg, kContiguous)
+REF_SYMBOL(PCR5_reg_single, float, kPCR5Reg, kContiguous)
+REF_SYMBOL(PCR4_reg_single, float, kPCR4Reg, kContiguous)
+REF_SYMBOL(WM5_shmem_double, double, kWM5Shmem, kContiguous)
+REF_SYMBOL(WM4_shmem_double, double, kWM4Shmem, kContiguous)
+REF_SYMBOL(PCR5_shmem_double, double, kPCR5Shmem, kContiguous)
+REF_SYMBOL(PCR4_shmem_double, double, kPCR4Shmem, kContiguous)
+REF_SYMBOL(WM5_reg_double, double, kWM5Reg, kContiguous)
+REF_SYMBOL(WM4_reg_double, double, kWM4Reg, kContiguous)
+REF_SYMBOL(PCR5_reg_double, double, kPCR5Reg, kContiguous)
+REF_SYMBOL(PCR4_reg_double, double, kPCR4Reg, kContiguous)
+REF_SYMBOL(CR_reg4_double, double, kCRReg4, kContiguous)
+REF_SYMBOL(CR_reg8_double, double, kCRReg8, kContiguous)
+REF_SYMBOL(CR_reg16_double, double, kCRReg16, kContiguous)
+REF_SYMBOL(CR_reg4_single, float, kCRReg4, kContiguous)
+REF_SYMBOL(CR_reg8_single, float, kCRReg8, kContiguous)
+REF_SYMBOL(CR_reg16_single, float, kCRReg16, kContiguous)
+REF_SYMBOL(BPLG_WM, float, kBPLGWM, kPacked)
+REF_SYMBOL(BPLG_LF, float, kBPLGLF, kPacked)
+REF_SYMBOL(BPLG_CR, float, kBPLGCR, kPacked)
+REF_SYMBOL(BPLG_PCR, float, kBPLGPCR, kPacked)
+
+int CR_init_single(float *, float *, float *, float *, int n_eqt, int n_batch) {
+    (void)n_eqt; (void)n_batch;
+    return 0;
+}
+int CR_init_double(double *, double *, double *, double *, int n_eqt, int n_batch) {
+    (void)n_eqt; (void)n_batch;
+    return 0;
+}
+void CR_final(void) {
+    if (g_cr_buf) (void)hipFree(g_cr_buf);
+    g_cr_buf = nullptr;
+}
+
+int cuSparse_init_single(float *a, float *b, float *c, float *d, int n_eqt, int n_batch) {
+    return sparse_init<float>(a, b, c, d, n_eqt, n_batch);
+}
+int cuSparse_init_double(double *a, double *b, double *c, double *d, int n_eqt, int n_batch) {
+    return sparse_init<double>(a, b, c, d, n_eqt, n_batch);
+}
+void cuSparse_final(void) {
+    if (g_sparse_buf) (void)hipFree(g_sparse_buf);
+    g_sparse_buf = nullptr;
+    g_sparse_bytes = 0;
+    if (g_sparse) hipsparseDestroy(g_sparse);
+    g_sparse = nullptr;
+}
+void cuSparse_single(float *a, float *b, float *c, float *d, int n_eqt, int n_batch) {
+    int rc = sparse_solve<float>(a, b, c, d, n_eqt, n_batch);
+    if (rc == TRIDIAG_OK) (void)hipDeviceSynchronize();
+    else fprintf(stderr, "tridiag_amd: cuSparse_single failed: %s\n", g_last_error.c_str());
+}
+void cuSparse_double(double *a, double *b, double *c, double *d, int n_eqt, int n_batch) {
+    int rc = sparse_solve<double>(a, b, c, d, n_eqt, n_batch);
+    if (rc == TRIDIAG_OK) (void)hipDeviceSynchronize();
+    else fprintf(stderr, "tridiag_amd: cuSparse_double failed: %s\n", g_last_error.c_str());
+}
+
+}  // extern "C"

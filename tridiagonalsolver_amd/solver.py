@@ -1,0 +1,133 @@
+"""Torch-facing entry points over the C-ABI (device memory and streams come from torch;
+all arithmetic happens in the HIP kernels of libtridiag_amd.so)."""
+import ctypes
+
+import torch
+
+from ._capi import (ALGO_AUTO, CONTIGUOUS, DOUBLE, INTERLEAVED, PACKED, SINGLE, check, lib)
+
+
+def _precision(t):
+    if t.dtype == torch.float32:
+        return SINGLE
+    if t.dtype == torch.float64:
+        return DOUBLE
+    raise TypeError("only float32 and float64 systems are supported, got %s" % t.dtype)
+
+
+def _stream_handle(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _check_tensors(ts):
+    t0 = ts[0]
+    if not t0.is_cuda:
+        raise ValueError("tensors must be on a GPU device")
+    for t in ts:
+        if t.dtype != t0.dtype or t.device != t0.device:
+            raise ValueError("all tensors must share dtype and device")
+        if not t.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+
+
+def _shape(t, layout, n, batch):
+    if n is not None and batch is not None:
+        return int(n), int(batch)
+    if t.dim() != 2:
+        raise ValueError("pass n and batch for 1-D tensors")
+    if layout == INTERLEAVED:
+        return t.shape[0], t.shape[1]
+    if layout == PACKED:
+        return t.shape[1] // 4, t.shape[0]
+    return t.shape[1], t.shape[0]
+
+
+def solve(a, b, c, d, algo=ALGO_AUTO, layout=CONTIGUOUS, n=None, batch=None):
+    """Solves in place: x is written to d; c may be clobbered (naive Thomas writes c').
+
+    layout 0: (batch, n) per-system rows; layout 2: (n, batch) interleaved;
+    layout 1: one (batch, 4n) packed buffer passed as `a` (b, c, d ignored).
+    """
+    ts = [a] if layout == PACKED else [a, b, c, d]
+    _check_tensors(ts)
+    n, batch = _shape(a if layout == PACKED else d, layout, n, batch)
+    prec = _precision(a)
+    args = [_ptr(a)] + ([_ptr(a)] * 3 if layout == PACKED else [_ptr(b), _ptr(c), _ptr(d)])
+    check(lib().tridiag_solve(prec, algo, layout, n, batch, *args, _stream_handle(a.device)))
+    return a if layout == PACKED else d
+
+
+def solve_ragged(a, b, c, d, offsets, work=None):
+    """Ragged batch: system s = rows [offsets[s], offsets[s+1]) of the 1-D tensors.
+    `offsets` is an int64 CPU tensor or list; x is written to d."""
+    _check_tensors([a, b, c, d])
+    off_host = torch.as_tensor(offsets, dtype=torch.int64).contiguous().cpu()
+    nsys = off_host.numel() - 1
+    off_dev = off_host.to(a.device, non_blocking=False)
+    if work is None:
+        work = torch.empty(int(lib().tridiag_ragged_work_size(nsys)), dtype=torch.uint8,
+                           device=a.device)
+    check(lib().tridiag_solve_ragged(_precision(a), nsys, ctypes.c_void_p(off_host.data_ptr()),
+                                     _ptr(off_dev), _ptr(a), _ptr(b), _ptr(c), _ptr(d),
+                                     _ptr(work), _stream_handle(a.device)))
+    return d
+
+
+def solve_adjoint(a, b, c, x, g, layout=CONTIGUOUS, n=None, batch=None):
+    """Returns (ga, gb, gc, gd) = dL/d(a, b, c, d) for dL/dx = g."""
+    _check_tensors([a, b, c, x, g])
+    n, batch = _shape(x, layout, n, batch)
+    outs = [torch.empty_like(x) for _ in range(4)]
+    check(lib().tridiag_adjoint(_precision(a), layout, n, batch, _ptr(a), _ptr(b), _ptr(c),
+                                _ptr(x), _ptr(g), *(_ptr(o) for o in outs),
+                                _stream_handle(a.device)))
+    return tuple(outs)
+
+
+def solve_fwd_bwd(a, b, c, d, g, layout=CONTIGUOUS, n=None, batch=None):
+    """Fused forward + adjoint for a known upstream gradient g: returns
+    (x, ga, gb, gc, gd)."""
+    _check_tensors([a, b, c, d, g])
+    n, batch = _shape(d, layout, n, batch)
+    x = torch.empty_like(d)
+    outs = [torch.empty_like(d) for _ in range(4)]
+    check(lib().tridiag_solve_fwd_bwd(_precision(a), layout, n, batch, _ptr(a), _ptr(b),
+                                      _ptr(c), _ptr(d), _ptr(g), _ptr(x),
+                                      *(_ptr(o) for o in outs), _stream_handle(a.device)))
+    return (x,) + tuple(outs)
+
+
+class TridiagonalSolve(torch.autograd.Function):
+    """x = A^{-1} d with A = tridiag(a, b, c); a_0 and c_{n-1} are not part of A.
+
+    Forward runs the on-chip solver on copies (inputs are not clobbered) and saves
+    only a, b, c and x.  Backward solves A^T lambda = dL/dx with the adjoint
+    kernel, which recomputes its own elimination, and returns
+    dL/da = -lambda x_{i-1}, dL/db = -lambda x, dL/dc = -lambda x_{i+1}, dL/dd = lambda.
+    """
+
+    @staticmethod
+    def forward(ctx, a, b, c, d, layout=CONTIGUOUS, algo=ALGO_AUTO):
+        n, batch = _shape(d, layout, None, None)
+        x = d.contiguous().clone()
+        cc = c.contiguous().clone() if algo == 1 else c.contiguous()
+        solve(a.contiguous(), b.contiguous(), cc, x, algo=algo, layout=layout, n=n, batch=batch)
+        ctx.layout = layout
+        ctx.save_for_backward(a, b, c, x)
+        return x
+
+    @staticmethod
+    def backward(ctx, gx):
+        a, b, c, x = ctx.saved_tensors
+        ga, gb, gc, gd = solve_adjoint(a.contiguous(), b.contiguous(), c.contiguous(), x,
+                                       gx.contiguous(), layout=ctx.layout)
+        return ga, gb, gc, gd, None, None
+
+
+def tridiagonal_solve(a, b, c, d, layout=CONTIGUOUS, algo=ALGO_AUTO):
+    """Differentiable solve: returns x, gradients flow to a, b, c, d."""
+    return TridiagonalSolve.apply(a, b, c, d, layout, algo)
