@@ -29,7 +29,8 @@ REFERENCE_SYMBOLS = [
 EXTENDED_SYMBOLS = [
     "tridiag_algo_count", "tridiag_algo_name", "tridiag_algo_layout", "tridiag_solve",
     "tridiag_ragged_work_size", "tridiag_solve_ragged", "tridiag_adjoint",
-    "tridiag_solve_fwd_bwd", "tridiag_last_error", "tridiag_plan",
+    "tridiag_solve_fwd_bwd", "tridiag_last_error", "tridiag_plan", "tridiag_tile_count",
+    "tridiag_tile_info", "tridiag_solve_tile",
 ]
 
 
@@ -72,6 +73,13 @@ def lib():
     L.tridiag_plan.argtypes = [ci, ci, ci, i64, i64, ctypes.POINTER(ci), ctypes.POINTER(ci),
                                ctypes.POINTER(ci)]
     L.tridiag_plan.restype = ci
+    L.tridiag_tile_count.argtypes = [ci]
+    L.tridiag_tile_count.restype = ci
+    pi = ctypes.POINTER(ci)
+    L.tridiag_tile_info.argtypes = [ci, ci, pi, pi, pi, pi, pi, pi, ctypes.POINTER(i64), pi, pi]
+    L.tridiag_tile_info.restype = ci
+    L.tridiag_solve_tile.argtypes = [ci, ci, ci, i64, i64, vp, vp, vp, vp, vp]
+    L.tridiag_solve_tile.restype = ci
     _lib = L
     return L
 
@@ -93,3 +101,19 @@ def plan(precision, algo, layout, n, batch):
     check(lib().tridiag_plan(precision, algo, layout, n, batch, ctypes.byref(M), ctypes.byref(NW),
                              ctypes.byref(S)))
     return dict(M=M.value, NW=NW.value, S=S.value)
+
+
+def tiles(precision):
+    """All compiled on-chip kernel configurations of a precision."""
+    L = lib()
+    out = []
+    for i in range(L.tridiag_tile_count(precision)):
+        v = [ctypes.c_int() for _ in range(6)]
+        cap, big, pad = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        check(L.tridiag_tile_info(precision, i, *(ctypes.byref(x) for x in v), ctypes.byref(cap),
+                                  ctypes.byref(big), ctypes.byref(pad)))
+        M, NW, S, il, vec, alds = (x.value for x in v)
+        out.append(dict(index=i, M=M, NW=NW, S=S, interleaved=bool(il), vec=bool(vec),
+                        lds_alpha=bool(alds), cap=cap.value, wide_addr=bool(big.value),
+                        padded=bool(pad.value)))
+    return out

@@ -1,0 +1,159 @@
+"""Autotuner: times every compiled on-chip kernel configuration for a
+(precision, layout, n, batch) workload on the current GPU and keeps the fastest
+one whose answer matches the FP64 reference solution.
+
+    python -m tridiagonalsolver_amd.autotune --prec 0 1 --layout 2 --n 4096 --batch 4096 \
+        --out results.json [--table tuning/tuning_gfx950.txt]
+
+The table file holds one line per bucket,
+    precision layout family n_max M NW S vec lds_alpha
+(family 0 = the default on-chip solver)
+and is read by libtridiag_amd.so at first use (TRIDIAG_TUNING_FILE overrides its
+location): for a solve of n rows the first bucket with n <= n_max whose kernel
+exists is used.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import _capi
+
+TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
+DEFAULT_TABLE = os.path.join(TUNING_DIR, "tuning_gfx950.txt")
+
+
+def _log(*a):
+    print("[autotune]", *a, file=sys.stderr, flush=True)
+
+
+def _inputs(torch, prec, layout, n, batch, seed, nsets):
+    """Seeded diagonally dominant systems (a, c ~ U[-1,1], b = |a|+|c|+U[1,2])."""
+    dtype = torch.float32 if prec == 0 else torch.float64
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    sets = []
+    shape = (batch, n) if layout == 0 else (n, batch)
+    for _ in range(nsets):
+        a = torch.rand(shape, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+        c = torch.rand(shape, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+        b = a.abs() + c.abs() + 1 + torch.rand(shape, generator=g, device="cuda",
+                                               dtype=torch.float64)
+        d = torch.rand(shape, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+        sets.append([t.to(dtype).contiguous() for t in (a, b, c, d)])
+    return sets
+
+
+def _reference(sets, prec, layout, n, batch):
+    """FP64 answer of the first set from the default (validated) solver path."""
+    import torch
+    a, b, c, d = (t.to(torch.float64) for t in sets[0])
+    x = d.clone()
+    _capi.check(_capi.lib().tridiag_solve(1, 9, layout, n, batch, ctypes.c_void_p(a.data_ptr()),
+                                          ctypes.c_void_p(b.data_ptr()),
+                                          ctypes.c_void_p(c.data_ptr()),
+                                          ctypes.c_void_p(x.data_ptr()), None))
+    return x
+
+
+def sweep(prec, layout, n, batch, reps=20, warmup=3, nsets=3, seed=1):
+    import torch
+    L = _capi.lib()
+    tiles = [t for t in _capi.tiles(prec) if t["interleaved"] == (layout == 2)
+             and t["cap"] >= n and not t["wide_addr"] and t["padded"] == (t["cap"] != n)]
+    sets = _inputs(torch, prec, layout, n, batch, seed, nsets)
+    xref = _reference(sets, prec, layout, n, batch)
+    pristine = [s[3].clone() for s in sets]
+    w = 4 if prec == 0 else 8
+    out = []
+    for t in tiles:
+        if t["vec"] and layout == 2:
+            continue
+        stream = torch.cuda.current_stream()
+        ptrs = [[ctypes.c_void_p(x.data_ptr()) for x in s] for s in sets]
+
+        def run(k):
+            return L.tridiag_solve_tile(prec, t["index"], layout, n, batch, *ptrs[k % nsets],
+                                        ctypes.c_void_p(stream.cuda_stream))
+        rc = run(0)
+        if rc != 0:
+            _log("skip", t, L.tridiag_last_error().decode())
+            continue
+        torch.cuda.synchronize()
+        x = sets[0][3].to(torch.float64)
+        err = float(((x - xref).abs().max() / xref.abs().max()).item())
+        for k in range(warmup):
+            run(k)
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(reps)]
+        for k in range(reps):
+            ev[k][0].record(stream)
+            run(k)
+            ev[k][1].record(stream)
+        torch.cuda.synchronize()
+        ms = sorted(e0.elapsed_time(e1) for e0, e1 in ev)
+        med = ms[len(ms) // 2]
+        gbs = 5 * w * n * batch / (med * 1e-3) / 1e9
+        rec = dict(t, ms=med, ms_min=ms[0], gbs=gbs, relerr=err)
+        out.append(rec)
+        _log("prec %d layout %d n %d: M=%d NW=%d S=%d vec=%d alds=%d  %.4f ms  %.0f GB/s  err %.2e"
+             % (prec, layout, n, t["M"], t["NW"], t["S"], t["vec"], t["lds_alpha"], med, gbs, err))
+        for s, p in zip(sets, pristine):      # in-place solves changed d; restore
+            s[3].copy_(p)
+    return out
+
+
+def best(results, tol):
+    ok = [r for r in results if r["relerr"] <= tol]
+    return min(ok, key=lambda r: r["ms"]) if ok else None
+
+
+def write_table(path, rows):
+    """rows: list of (prec, layout, n_max, entry) tuples."""
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        f.write("# precision layout family n_max M NW S vec lds_alpha  (written by autotune.py)\n")
+        for prec, layout, nmax, e in sorted(rows, key=lambda r: (r[0], r[1], r[2])):
+            f.write("%d %d 0 %d %d %d %d %d %d\n" % (prec, layout, nmax, e["M"], e["NW"], e["S"],
+                                                     int(e["vec"]), int(e["lds_alpha"])))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--prec", type=int, nargs="+", default=[0, 1])
+    ap.add_argument("--layout", type=int, nargs="+", default=[2])
+    ap.add_argument("--n", type=int, nargs="+", default=[4096])
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--unknowns", type=int, default=0,
+                    help="if set, batch = unknowns // n for each n (constant work per point)")
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--table", default=None, help="write the winners as a tuning table")
+    args = ap.parse_args(argv)
+    allres, rows = [], []
+    t0 = time.time()
+    for prec in args.prec:
+        for layout in args.layout:
+            for n in args.n:
+                batch = args.unknowns // n if args.unknowns else args.batch
+                res = sweep(prec, layout, n, batch, reps=args.reps)
+                allres.append(dict(prec=prec, layout=layout, n=n, batch=batch, results=res))
+                b = best(res, 1e-5 if prec == 0 else 1e-12)
+                if b:
+                    rows.append((prec, layout, n, b))
+                    _log("best prec %d layout %d n %d -> M=%d NW=%d S=%d vec=%d alds=%d %.4f ms"
+                         % (prec, layout, n, b["M"], b["NW"], b["S"], b["vec"], b["lds_alpha"],
+                            b["ms"]))
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(allres, f, indent=1)
+    if args.table:
+        write_table(args.table, rows)
+    _log("done in %.1f s" % (time.time() - t0))
+
+
+if __name__ == "__main__":
+    main()

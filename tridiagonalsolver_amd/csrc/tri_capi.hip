@@ -6,9 +6,13 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/tridiagonal.h"
 #include "tri_host.hpp"
@@ -46,8 +50,52 @@ static const char *const kNamesDouble[] = {
 
 static int algo_count(int prec) { return prec == TRIDIAG_SINGLE ? 18 : 14; }
 
+// ---------------------------------------------------------------- tuning table
+// Lines "precision layout family n_max M NW S vec lds_alpha" written by the
+// autotuner (tridiagonalsolver_amd/autotune.py).  Read once, from
+// $TRIDIAG_TUNING_FILE or <library dir>/../tuning/tuning_gfx950.txt.
+struct TuneRow {
+    int prec, layout, family;
+    long long nmax;
+    int M, NW, S, vec, alds;
+};
+static std::vector<TuneRow> g_tune;
+static std::once_flag g_tune_once;
+
+static void load_tuning() {
+    std::string path;
+    if (const char *e = getenv("TRIDIAG_TUNING_FILE")) {
+        path = e;
+    } else {
+        Dl_info info;
+        if (dladdr((void *)&load_tuning, &info) && info.dli_fname) {
+            std::string so = info.dli_fname;
+            size_t k = so.rfind('/');
+            std::string dir = (k == std::string::npos) ? "." : so.substr(0, k);
+            path = dir + "/../tuning/tuning_gfx950.txt";
+        }
+    }
+    if (path.empty()) return;
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) return;
+    char line[256];
+    while (fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        TuneRow r;
+        if (sscanf(line, "%d %d %d %lld %d %d %d %d %d", &r.prec, &r.layout, &r.family, &r.nmax,
+                   &r.M, &r.NW, &r.S, &r.vec, &r.alds) == 9)
+            g_tune.push_back(r);
+    }
+    fclose(f);
+    std::stable_sort(g_tune.begin(), g_tune.end(),
+                     [](const TuneRow &x, const TuneRow &y) { return x.nmax < y.nmax; });
+}
+
 // ---------------------------------------------------------------- planning
-template <typename T> static int pref_M() { return sizeof(T) == 4 ? 32 : 16; }
+template <typename T> static int pref_M(bool il) {
+    if (il) return sizeof(T) == 4 ? 32 : 16;
+    return sizeof(T) == 4 ? 4 : 8;
+}
 template <typename T> static int pref_S_il() { return 32 / (int)sizeof(T); }
 
 // Rows per lane fixed by the algorithm (register-packed CR: R rows per lane).
@@ -61,27 +109,58 @@ static int forced_M(int algo) {
 }
 
 template <typename T>
-static const TileEntry<T> *choose_tile(int algo, int layout, int64_t n, bool vec_ok) {
+static const TileEntry<T> *find_tile(int M, int NW, int S, bool il, bool vec, bool alds, bool pad,
+                                     bool big) {
+    TileTable<T> tab = tile_table<T>();
+    for (int i = 0; i < tab.count; ++i) {
+        const TileEntry<T> &e = tab.entries[i];
+        if (e.M == M && e.NW == NW && e.S == S && e.IL == il && e.VEC == vec && e.ALDS == alds &&
+            e.PAD == pad && e.BIG == big)
+            return &e;
+    }
+    return nullptr;
+}
+
+template <typename T>
+static const TileEntry<T> *choose_tile(int algo, int layout, int64_t n, bool vec_ok,
+                                       bool big = false) {
     TileTable<T> tab = tile_table<T>();
     const bool il = layout == kInterleaved;
     const int fM = forced_M(algo);
+    if (!fM && !big) {
+        std::call_once(g_tune_once, load_tuning);
+        const int prec = sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE;
+        for (const TuneRow &r : g_tune) {
+            if (r.prec != prec || r.layout != layout || r.family != 0 || n > r.nmax) continue;
+            if (r.vec && !vec_ok) continue;
+            const int nws = il ? r.NW : r.NW / r.S;
+            const int64_t cap = (int64_t)(il ? 64 / r.S : 64) * nws * r.M;
+            if (cap < n || cap > 2 * n) continue;   // bucket too coarse for this n
+            const TileEntry<T> *e = find_tile<T>(r.M, r.NW, r.S, il, r.vec, r.alds, cap != n, false);
+            if (e) return e;
+        }
+    }
     const TileEntry<T> *best = nullptr;
-    long long best_key[5] = {0, 0, 0, 0, 0};
+    long long best_key[6] = {0, 0, 0, 0, 0, 0};
     for (int i = 0; i < tab.count; ++i) {
         const TileEntry<T> &e = tab.entries[i];
-        if (e.IL != il || e.cap < n) continue;
+        if (e.IL != il || e.cap < n || e.BIG != big) continue;
+        if (!e.BIG && e.PAD != (e.cap != n)) continue;   // exact tiles take the unpredicated kernel
         if (e.VEC && !vec_ok) continue;
         if (fM && e.M != fM) continue;
         const int nws = il ? e.NW : e.NW / e.S;
-        long long key[5] = {
+        // Register-heavy segments keep one coefficient in LDS.
+        const bool want_alds = e.M * (int)sizeof(T) >= 256;
+        long long key[6] = {
             il ? (e.S == pref_S_il<T>() ? 0 : 1) : 0,  // 32-byte row groups in IL
             (long long)e.cap,                          // least padding
             il ? 0 : (long long)nws,                   // fewest waves per system
-            (long long)std::abs(e.M - pref_M<T>()),    // moderate register use
+            (long long)std::abs(e.M - pref_M<T>(il)),  // measured sweet spot per layout
+            e.ALDS == want_alds ? 0 : 1,
             e.VEC ? 0 : 1};
-        if (!best || std::lexicographical_compare(key, key + 5, best_key, best_key + 5)) {
+        if (!best || std::lexicographical_compare(key, key + 6, best_key, best_key + 6)) {
             best = &e;
-            std::copy(key, key + 5, best_key);
+            std::copy(key, key + 6, best_key);
         }
     }
     return best;
@@ -89,6 +168,14 @@ static const TileEntry<T> *choose_tile(int algo, int layout, int64_t n, bool vec
 
 template <typename T> static bool aligned16(const void *p) {
     return (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+}
+
+// Byte extent of one coefficient array if 32-bit buffer offsets can address it.
+template <typename T> static int32_t buffer_extent(int64_t n, int64_t batch, int64_t sstride,
+                                                   int layout) {
+    const int64_t elems = (layout == kInterleaved) ? n * batch : batch * sstride;
+    const int64_t bytes = elems * (int64_t)sizeof(T);
+    return bytes < (int64_t)0x7fffff00 ? (int32_t)bytes : 0;
 }
 
 template <typename T>
@@ -102,11 +189,31 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
     if (batch == 0) return TRIDIAG_OK;
     int64_t sstride = n, rstride = 1;
     if (layout == kPacked) {
-        b = a + n; c = a + 2 * n; d = a + 3 * n;
         sstride = 4 * n;
     } else if (layout == kInterleaved) {
         sstride = 1;
         rstride = batch;
+    }
+    if (algo != kThomas && layout != kInterleaved &&
+        buffer_extent<T>(n, batch, sstride, layout) == 0) {
+        // Contiguous systems: solve in chunks whose arrays stay below 2 GiB so
+        // the kernels keep 32-bit buffer offsets.
+        const int64_t per = std::max<int64_t>(1, ((int64_t)0x7fffff00 / (int64_t)sizeof(T)) / sstride);
+        if (per < batch) {
+            for (int64_t s0 = 0; s0 < batch; s0 += per) {
+                const int64_t cnt = std::min(per, batch - s0);
+                const int64_t o = s0 * sstride;
+                int rc = solve_impl<T>(algo, layout, n, cnt, a + o,
+                                       layout == kPacked ? nullptr : b + o,
+                                       layout == kPacked ? nullptr : c + o,
+                                       layout == kPacked ? nullptr : d + o, st);
+                if (rc != TRIDIAG_OK) return rc;
+            }
+            return TRIDIAG_OK;
+        }
+    }
+    if (layout == kPacked) {
+        b = a + n; c = a + 2 * n; d = a + 3 * n;
     }
     if (algo == kThomas) {
         hipError_t e = (sizeof(T) == 4)
@@ -119,13 +226,14 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
     const int V = 16 / (int)sizeof(T);
     const bool vec_ok = layout != kInterleaved && (sstride % V) == 0 && aligned16<T>(a) &&
                         aligned16<T>(b) && aligned16<T>(c) && aligned16<T>(d);
-    const TileEntry<T> *e = choose_tile<T>(algo, layout, n, vec_ok);
+    const int32_t bb = buffer_extent<T>(n, batch, sstride, layout);
+    const TileEntry<T> *e = choose_tile<T>(algo, layout, n, vec_ok, bb == 0);
     if (!e) {
         return fail(TRIDIAG_EUNSUPPORTED,
                     "n_eqt=" + std::to_string(n) + " exceeds the on-chip capacity of algorithm " +
                         std::to_string(algo) + " in layout " + std::to_string(layout));
     }
-    TileParams<T> p{a, b, c, d, n, batch, sstride, rstride};
+    TileParams<T> p{a, b, c, d, n, batch, sstride, rstride, bb};
     hipError_t err = e->launch(p, st);
     return err == hipSuccess ? TRIDIAG_OK : hip_fail(err, "tile launch");
 }
@@ -187,6 +295,55 @@ template <typename T> static void ref_call(int algo, int layout, T *a, T *b, T *
                 sizeof(T) == 4 ? kNamesSingle[algo] : kNamesDouble[algo], algo,
                 g_last_error.c_str());
     }
+}
+
+template <typename T>
+static int tile_info_impl(int index, int *M, int *NW, int *S, int *il, int *vec, int *alds,
+                          int64_t *cap, int *big, int *pad) {
+    TileTable<T> tab = tile_table<T>();
+    if (index < 0 || index >= tab.count) return fail(TRIDIAG_EINVAL, "tile index out of range");
+    const TileEntry<T> &e = tab.entries[index];
+    if (M) *M = e.M;
+    if (NW) *NW = e.NW;
+    if (S) *S = e.S;
+    if (il) *il = e.IL;
+    if (vec) *vec = e.VEC;
+    if (alds) *alds = e.ALDS;
+    if (cap) *cap = e.cap;
+    if (big) *big = e.BIG;
+    if (pad) *pad = e.PAD;
+    return TRIDIAG_OK;
+}
+
+template <typename T>
+static int solve_tile_impl(int index, int layout, int64_t n, int64_t batch, T *a, T *b, T *c,
+                           T *d, hipStream_t st) {
+    TileTable<T> tab = tile_table<T>();
+    if (index < 0 || index >= tab.count) return fail(TRIDIAG_EINVAL, "tile index out of range");
+    const TileEntry<T> &e = tab.entries[index];
+    if (n < 1 || batch < 0 || layout < 0 || layout > 2) return fail(TRIDIAG_EINVAL, "bad shape");
+    if (!a || (layout != kPacked && (!b || !c || !d))) return fail(TRIDIAG_EINVAL, "null pointer");
+    if (e.IL != (layout == kInterleaved)) return fail(TRIDIAG_EINVAL, "tile/layout mismatch");
+    if (e.cap < n) return fail(TRIDIAG_EUNSUPPORTED, "n_eqt exceeds the tile capacity");
+    if (!e.PAD && e.cap != n) return fail(TRIDIAG_EINVAL, "unpadded tile needs n_eqt == capacity");
+    if (batch == 0) return TRIDIAG_OK;
+    int64_t sstride = n, rstride = 1;
+    if (layout == kPacked) {
+        b = a + n; c = a + 2 * n; d = a + 3 * n;
+        sstride = 4 * n;
+    } else if (layout == kInterleaved) {
+        sstride = 1;
+        rstride = batch;
+    }
+    const int V = 16 / (int)sizeof(T);
+    if (e.VEC && !((sstride % V) == 0 && aligned16<T>(a) && aligned16<T>(b) && aligned16<T>(c) &&
+                   aligned16<T>(d)))
+        return fail(TRIDIAG_EINVAL, "vector tile needs 16-byte aligned rows");
+    const int32_t bb = buffer_extent<T>(n, batch, sstride, layout);
+    if (bb == 0 && !e.BIG) return fail(TRIDIAG_EUNSUPPORTED, "arrays of 2 GiB or more need a BIG tile");
+    TileParams<T> p{a, b, c, d, n, batch, sstride, rstride, bb};
+    hipError_t err = e.launch(p, st);
+    return err == hipSuccess ? TRIDIAG_OK : hip_fail(err, "tile launch");
 }
 
 }  // namespace tri
@@ -263,6 +420,35 @@ int tridiag_plan(int precision, int algo, int layout, int64_t n_eqt, int64_t n_b
     if (NW) *NW = nw;
     if (S) *S = s;
     return TRIDIAG_OK;
+}
+
+int tridiag_tile_count(int precision) {
+    if (precision == TRIDIAG_SINGLE) return tile_table<float>().count;
+    if (precision == TRIDIAG_DOUBLE) return tile_table<double>().count;
+    return 0;
+}
+
+int tridiag_tile_info(int precision, int index, int *M, int *NW, int *S, int *interleaved,
+                      int *vec, int *lds_alpha, int64_t *cap, int *wide_addr, int *padded) {
+    if (precision == TRIDIAG_SINGLE)
+        return tile_info_impl<float>(index, M, NW, S, interleaved, vec, lds_alpha, cap, wide_addr,
+                                     padded);
+    if (precision == TRIDIAG_DOUBLE)
+        return tile_info_impl<double>(index, M, NW, S, interleaved, vec, lds_alpha, cap, wide_addr,
+                                      padded);
+    return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
+}
+
+int tridiag_solve_tile(int precision, int index, int layout, int64_t n_eqt, int64_t n_batch,
+                       void *a, void *b, void *c, void *d, void *stream) {
+    g_last_error.clear();
+    if (precision == TRIDIAG_SINGLE)
+        return solve_tile_impl<float>(index, layout, n_eqt, n_batch, (float *)a, (float *)b,
+                                      (float *)c, (float *)d, (hipStream_t)stream);
+    if (precision == TRIDIAG_DOUBLE)
+        return solve_tile_impl<double>(index, layout, n_eqt, n_batch, (double *)a, (double *)b,
+                                       (double *)c, (double *)d, (hipStream_t)stream);
+    return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
 }
 
 int64_t tridiag_ragged_work_size(int64_t n_sys) { return n_sys < 0 ? 0 : 64 * (n_sys + 1); }

@@ -57,26 +57,241 @@ template <typename T> struct TileParams {
     int64_t n, batch;
     int64_t sstride;  // distance between systems (IL: 1)
     int64_t rstride;  // distance between rows (IL: batch)
+    // Byte extent of each coefficient array when it is below 2^31 (buffer
+    // addressing with 32-bit offsets is allowed), else 0.
+    int32_t buf_bytes;
 };
 
-template <typename T, int M, int NW, int S, bool IL, bool VEC>
+template <typename T, int M, int NW, int S, bool IL, bool VEC, bool ALDS, bool BIG, bool PAD>
 struct TileShape {
     static_assert(M >= 2, "a lane segment needs at least two rows");
+    static constexpr int M_ = M;
     static constexpr int LS = IL ? S : 1;        // lane stride between segments of a system
     static constexpr int SW = 64 / LS;           // segments of one system per wave
     static constexpr int NWS = IL ? NW : NW / S; // waves per system
     static constexpr int P = SW * NWS;           // segments per system
     static constexpr int CAP = P * M;            // rows per system the tile can hold
     static constexpr int NI = (M > 2) ? (M - 2) : 1;
+    static constexpr int V = (VEC && !IL) ? Vec16<T>::N : 1;      // elements per load
+    // Prefetch ring: NBUF chunks of CH rows; at most (NBUF-1)*CH*4/V loads in
+    // flight per wave, below the 63 outstanding loads a wave can count.
+    static constexpr int CHR = (V > 1) ? 8 : (sizeof(T) == 4 ? 4 : 2);
+    static constexpr int CH = (M < CHR) ? M : CHR;
+    static constexpr int NC = M / CH;
+    static constexpr int NBUF_MAX = (NW >= 16) ? 2 : 3;   // 1024-lane groups: 128 VGPRs
+    static constexpr int NBUF = (NC >= NBUF_MAX) ? NBUF_MAX : NC;
     static_assert(IL ? (64 % S == 0) : (NW % S == 0), "bad tile shape");
+    static_assert(M % CH == 0 && CH % V == 0, "chunking must divide the segment");
 };
 
-template <typename T, int M, int NW, int S, bool IL, bool VEC>
+// ---------------------------------------------------------------- loaders
+// Fast path: every row of every lane lies inside the system (n == CAP) and each
+// array spans < 2^31 bytes.  Buffer loads address a row as one per-lane VGPR
+// offset plus a wave-uniform SGPR row offset, so in-flight loads cost no
+// address registers.
+template <typename T> __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+template <typename T> __device__ __forceinline__ void bstore(T x, __amdgpu_buffer_rsrc_t r, int vo,
+                                                            int so) {
+    if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, vo, so, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(
+            __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, x), r, vo, so, 0);
+}
+using u32x4 = __attribute__((ext_vector_type(4))) unsigned;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, int32_t bytes) {
+    // raw buffer (stride 0); word 3 = 0x00020000 selects 32-bit data format on gfx9
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
+}
+
+template <typename T, int CH, int V> struct FastLoader {
+    __amdgpu_buffer_rsrc_t ra, rb, rc, rd;
+    int vo;       // lane byte offset of its first row
+    int rowb;     // bytes between consecutive rows of one system
+    __device__ __forceinline__ void load(T (&q)[4][CH], int j0) const {
+        if constexpr (V == 1) {
+#pragma unroll
+            for (int v = 0; v < CH; ++v) {
+                const int so = (j0 + v) * rowb;
+                q[0][v] = bload<T>(ra, vo, so);
+                q[1][v] = bload<T>(rb, vo, so);
+                q[2][v] = bload<T>(rc, vo, so);
+                q[3][v] = bload<T>(rd, vo, so);
+            }
+        } else {
+#pragma unroll
+            for (int v0 = 0; v0 < CH; v0 += V) {
+                const int so = (j0 + v0) * rowb;
+                u32x4 xa = __builtin_amdgcn_raw_buffer_load_b128(ra, vo, so, 0);
+                u32x4 xb = __builtin_amdgcn_raw_buffer_load_b128(rb, vo, so, 0);
+                u32x4 xc = __builtin_amdgcn_raw_buffer_load_b128(rc, vo, so, 0);
+                u32x4 xd = __builtin_amdgcn_raw_buffer_load_b128(rd, vo, so, 0);
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    q[0][v0 + v] = reinterpret_cast<const T *>(&xa)[v];
+                    q[1][v0 + v] = reinterpret_cast<const T *>(&xb)[v];
+                    q[2][v0 + v] = reinterpret_cast<const T *>(&xc)[v];
+                    q[3][v0 + v] = reinterpret_cast<const T *>(&xd)[v];
+                }
+            }
+        }
+    }
+};
+
+// Padded tiles (n < CAP): the same buffer addressing, rows past the end of the
+// system are identity rows (0, 1, 0, 0) and are not read.
+template <typename T, int CH, int V> struct PredLoader {
+    __amdgpu_buffer_rsrc_t ra, rb, rc, rd;
+    int vo, rowb;
+    int64_t row0, n;
+    __device__ __forceinline__ void load(T (&q)[4][CH], int j0) const {
+#pragma unroll
+        for (int v0 = 0; v0 < CH; v0 += V) {
+            const int64_t r = row0 + j0 + v0;
+            const int so = (j0 + v0) * rowb;
+            if (V > 1 && r + V <= n) {
+                u32x4 xa = __builtin_amdgcn_raw_buffer_load_b128(ra, vo, so, 0);
+                u32x4 xb = __builtin_amdgcn_raw_buffer_load_b128(rb, vo, so, 0);
+                u32x4 xc = __builtin_amdgcn_raw_buffer_load_b128(rc, vo, so, 0);
+                u32x4 xd = __builtin_amdgcn_raw_buffer_load_b128(rd, vo, so, 0);
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    q[0][v0 + v] = reinterpret_cast<const T *>(&xa)[v];
+                    q[1][v0 + v] = reinterpret_cast<const T *>(&xb)[v];
+                    q[2][v0 + v] = reinterpret_cast<const T *>(&xc)[v];
+                    q[3][v0 + v] = reinterpret_cast<const T *>(&xd)[v];
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    if (r + v < n) {
+                        const int sov = so + v * rowb;
+                        q[0][v0 + v] = bload<T>(ra, vo, sov);
+                        q[1][v0 + v] = bload<T>(rb, vo, sov);
+                        q[2][v0 + v] = bload<T>(rc, vo, sov);
+                        q[3][v0 + v] = bload<T>(rd, vo, sov);
+                    } else {
+                        q[0][v0 + v] = 0; q[1][v0 + v] = 1; q[2][v0 + v] = 0; q[3][v0 + v] = 0;
+                    }
+                }
+            }
+        }
+    }
+};
+
+// Arrays of 2 GiB or more: 64-bit addressing, identity rows past the end.
+template <typename T, int CH, int V> struct SafeLoader {
+    const T *A, *B, *C, *D;
+    int64_t base, rs, row0, n;
+    bool valid;
+    __device__ __forceinline__ void load(T (&q)[4][CH], int j0) const {
+#pragma unroll
+        for (int v0 = 0; v0 < CH; v0 += V) {
+            const int64_t r = row0 + j0 + v0;
+            if (V > 1 && valid && r + V <= n) {
+                using VT = typename Vec16<T>::type;
+                const int64_t off = base + r;
+                VT xa = *reinterpret_cast<const VT *>(A + off);
+                VT xb = *reinterpret_cast<const VT *>(B + off);
+                VT xc = *reinterpret_cast<const VT *>(C + off);
+                VT xd = *reinterpret_cast<const VT *>(D + off);
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    q[0][v0 + v] = reinterpret_cast<const T *>(&xa)[v];
+                    q[1][v0 + v] = reinterpret_cast<const T *>(&xb)[v];
+                    q[2][v0 + v] = reinterpret_cast<const T *>(&xc)[v];
+                    q[3][v0 + v] = reinterpret_cast<const T *>(&xd)[v];
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const int64_t rr = r + v;
+                    if (valid && rr < n) {
+                        const int64_t off = base + rr * rs;
+                        q[0][v0 + v] = A[off];
+                        q[1][v0 + v] = B[off];
+                        q[2][v0 + v] = C[off];
+                        q[3][v0 + v] = D[off];
+                    } else {
+                        q[0][v0 + v] = 0; q[1][v0 + v] = 1; q[2][v0 + v] = 0; q[3][v0 + v] = 0;
+                    }
+                }
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------- lane sweep
+// Forward modified-Thomas over the lane's interior rows j = 1..M-2 with x_L kept
+// symbolic.  Leaves x_j = de_j - al_j x_L - ga_j x_{j+1} for every interior row
+// and accumulates x_1 = SD - SA x_L + PI x_R on the way:
+//   x_1 = sum_j pi_j (de_j - al_j x_L) + pi_{M-1} x_R,  pi_1 = 1, pi_{j+1} = -ga_j pi_j.
+template <typename T, int NI> struct SweepOut {
+    T ga[NI], de[NI];
+    T aL, bL, cL, dL, aR, bR, cR, dR;
+    T g, alp, del, SD, SA, PI;
+};
+
+template <typename T, class Sh, class Loader, class AlStore>
+__device__ __forceinline__ void lane_sweep(SweepOut<T, Sh::NI> &o, const Loader &ld,
+                                           int64_t row0, int64_t n, AlStore al_store) {
+    constexpr int CH = Sh::CH, NC = Sh::NC, NBUF = Sh::NBUF, M = Sh::M_;
+    o.aL = 0; o.bL = 1; o.cL = 0; o.dL = 0; o.aR = 0; o.bR = 1; o.cR = 0; o.dR = 0;
+    o.g = 0; o.alp = -1; o.del = 0;   // virtual row before the first interior row
+    o.SD = 0; o.SA = 0; o.PI = 1;
+    T buf[NBUF][4][CH];
+#pragma unroll
+    for (int p = 0; p < NBUF - 1; ++p) ld.load(buf[p], p * CH);
+#pragma unroll
+    for (int ck = 0; ck < NC; ++ck) {
+        if (ck + NBUF - 1 < NC) ld.load(buf[(ck + NBUF - 1) % NBUF], (ck + NBUF - 1) * CH);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int v = 0; v < CH; ++v) {
+            const int j = ck * CH + v;
+            const int64_t r = row0 + j;
+            T a = buf[ck % NBUF][0][v], b = buf[ck % NBUF][1][v];
+            T c = buf[ck % NBUF][2][v], d = buf[ck % NBUF][3][v];
+            a = (r == 0) ? T(0) : a;            // a_0 is not part of A
+            c = (r == n - 1) ? T(0) : c;        // nor is c_{n-1}
+            if (j == 0) {
+                o.aL = a; o.bL = b; o.cL = c; o.dL = d;
+            } else if (j == M - 1) {
+                o.aR = a; o.bR = b; o.cR = c; o.dR = d;
+            } else {
+                const T rr = rcp(b - a * o.g);
+                o.g = c * rr;
+                o.alp = -(a * o.alp) * rr;
+                o.del = (d - a * o.del) * rr;
+                o.SD += o.PI * o.del;
+                o.SA += o.PI * o.alp;
+                o.PI = -o.g * o.PI;
+                al_store(j - 1, o.alp);
+                o.ga[j - 1] = o.g;
+                o.de[j - 1] = o.del;
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// PAD = false: the system fills the tile exactly (n == CAP), loads are not
+// predicated.  PAD = true: n < CAP, rows past the end are identity rows.
+// BIG: 64-bit addressing for arrays of 2 GiB or more (always predicated).
+template <typename T, int M, int NW, int S, bool IL, bool VEC, bool ALDS, bool BIG, bool PAD>
 __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
-    using Sh = TileShape<T, M, NW, S, IL, VEC>;
-    constexpr int SW = Sh::SW, NWS = Sh::NWS, LS = Sh::LS;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    using Sh = TileShape<T, M, NW, S, IL, VEC, ALDS, BIG, PAD>;
+    constexpr int SW = Sh::SW, NWS = Sh::NWS, LS = Sh::LS, V = Sh::V, CH = Sh::CH;
+    constexpr int LANES = 64 * NW;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
     const int sl = IL ? (lane % S) : (wave / NWS);   // system inside the workgroup
     const int kw = IL ? (lane / S) : lane;           // segment inside the wave-block
     const int ws = IL ? wave : (wave % NWS);         // wave-block inside the system
@@ -85,100 +300,56 @@ __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
     if (IL) tile = xcd_group(blockIdx.x, gridDim.x, 128 / (S * (int)sizeof(T)));
     const int64_t sys = (int64_t)tile * S + sl;
     const bool valid = sys < prm.batch;
+    const int64_t sysc = valid ? sys : prm.batch - 1;   // loads of idle lanes stay in bounds
     const int64_t row0 = (int64_t)(ws * SW + kw) * M;
-    const int64_t base = valid ? sys * prm.sstride : 0;
+    const int64_t base = sysc * prm.sstride;
     const int64_t rs = prm.rstride;
-    const T *__restrict__ A = prm.a;
-    const T *__restrict__ B = prm.b;
-    const T *__restrict__ C = prm.c;
-    T *__restrict__ D = prm.d;
+    constexpr bool fast = !BIG && !PAD;
 
-    // ---------------------------------------------------------------- 1. sweep
-    T al[Sh::NI], ga[Sh::NI], de[Sh::NI];
-    T aL = 0, bL = 1, cL = 0, dL = 0, aR = 0, bR = 1, cR = 0, dR = 0;
-    T g = 0, alp = -1, del = 0;  // virtual row before the first interior row
+    // Interior coefficients; al lives in LDS when ALDS (it is only read back in
+    // the substitution), laid out [row][lane] so a wave's accesses are contiguous.
+    __shared__ T sh_al[ALDS ? Sh::NI : 1][ALDS ? LANES : 1];
+    T al_r[ALDS ? 1 : Sh::NI];
+    (void)al_r;
+    (void)sh_al;
+    auto al_store = [&](int j, T x) {
+        if constexpr (ALDS) sh_al[j][tid] = x;
+        else al_r[j] = x;
+    };
 
-    constexpr int V = (VEC && !IL) ? Vec16<T>::N : 1;
-    static_assert(M % V == 0, "vector width must divide M");
-#pragma unroll
-    for (int j0 = 0; j0 < M; j0 += V) {
-        T va[V], vb[V], vc[V], vd[V];
-        const int64_t r0 = row0 + j0;
-        if (V > 1 && valid && r0 + V <= n) {
-            using VT = typename Vec16<T>::type;
-            const int64_t off = base + r0;
-            VT xa = *reinterpret_cast<const VT *>(A + off);
-            VT xb = *reinterpret_cast<const VT *>(B + off);
-            VT xc = *reinterpret_cast<const VT *>(C + off);
-            VT xd = *reinterpret_cast<const VT *>(D + off);
-#pragma unroll
-            for (int v = 0; v < V; ++v) {
-                va[v] = reinterpret_cast<const T *>(&xa)[v];
-                vb[v] = reinterpret_cast<const T *>(&xb)[v];
-                vc[v] = reinterpret_cast<const T *>(&xc)[v];
-                vd[v] = reinterpret_cast<const T *>(&xd)[v];
-            }
+    SweepOut<T, Sh::NI> o;
+    const int32_t nb = BIG ? 0x7fffffff : prm.buf_bytes;
+    __amdgpu_buffer_rsrc_t rd_rsrc = make_rsrc(prm.d, nb);
+    const int vo = BIG ? 0 : (int)(((IL ? row0 * rs : row0) + base) * (int64_t)sizeof(T));
+    const int rowb = (int)(rs * (int64_t)sizeof(T));
+    if constexpr (BIG) {
+        SafeLoader<T, CH, V> ld{prm.a, prm.b, prm.c, prm.d, base, rs, row0, n, valid};
+        lane_sweep<T, Sh>(o, ld, row0, n, al_store);
+    } else {
+        __amdgpu_buffer_rsrc_t ra = make_rsrc(prm.a, nb), rb = make_rsrc(prm.b, nb),
+                               rc = make_rsrc(prm.c, nb);
+        if constexpr (fast) {
+            FastLoader<T, CH, V> ld{ra, rb, rc, rd_rsrc, vo, rowb};
+            lane_sweep<T, Sh>(o, ld, row0, n, al_store);
         } else {
-#pragma unroll
-            for (int v = 0; v < V; ++v) {
-                const int64_t r = r0 + v;
-                if (valid && r < n) {
-                    const int64_t off = base + r * rs;
-                    va[v] = A[off];
-                    vb[v] = B[off];
-                    vc[v] = C[off];
-                    vd[v] = D[off];
-                } else {
-                    va[v] = 0; vb[v] = 1; vc[v] = 0; vd[v] = 0;
-                }
-            }
-        }
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-            const int j = j0 + v;
-            const int64_t r = r0 + v;
-            T a = (r == 0) ? T(0) : va[v];          // a_0 is not part of A
-            T c = (r == n - 1) ? T(0) : vc[v];      // nor is c_{n-1}
-            T b = vb[v], d = vd[v];
-            if (j == 0) {
-                aL = a; bL = b; cL = c; dL = d;
-            } else if (j == M - 1) {
-                aR = a; bR = b; cR = c; dR = d;
-            } else {
-                T rr = rcp(b - a * g);
-                g = c * rr;
-                alp = -(a * alp) * rr;
-                del = (d - a * del) * rr;
-                al[j - 1] = alp;
-                ga[j - 1] = g;
-                de[j - 1] = del;
-            }
+            PredLoader<T, CH, V> ld{ra, rb, rc, rd_rsrc, vo, rowb, row0, n};
+            lane_sweep<T, Sh>(o, ld, row0, n, al_store);
         }
     }
-    // Row L from x_1 = DD - AA x_L - GG x_R (backward over the interior).
-    T DD = 0, AA = 0, GG = -1;
-    if constexpr (M > 2) {
-        DD = de[M - 3]; AA = al[M - 3]; GG = ga[M - 3];
-#pragma unroll
-        for (int j = M - 4; j >= 0; --j) {
-            DD = de[j] - ga[j] * DD;
-            AA = al[j] - ga[j] * AA;
-            GG = -ga[j] * GG;
-        }
-    }
+
     // Reduced rows of the segment:
     //   L: aL u + DLr x_L + ELr x_R = FLr ;  R: ARr x_L + DRr x_R + cR v = FRr
-    const T DLr = bL - cL * AA, ELr = -(cL * GG), FLr = dL - cL * DD;
-    const T ARr = -(aR * alp), DRr = bR - aR * g, FRr = dR - aR * del;
+    const T DLr = o.bL - o.cL * o.SA, ELr = o.cL * o.PI, FLr = o.dL - o.cL * o.SD;
+    const T ARr = -(o.aR * o.alp), DRr = o.bR - o.aR * o.g, FRr = o.dR - o.aR * o.del;
     Outer<T> oL, oR;
     {
         T inv = rcp(DLr * DRr - ELr * ARr);
         oL.r = (DRr * FLr - ELr * FRr) * inv;
-        oL.p = (DRr * aL) * inv;
-        oL.q = -(ELr * cR) * inv;
+        oL.p = (DRr * o.aL) * inv;
+        oL.q = -(ELr * o.cR) * inv;
         oR.r = (DLr * FRr - ARr * FLr) * inv;
-        oR.p = -(ARr * aL) * inv;
-        oR.q = (DLr * cR) * inv;
+        oR.p = -(ARr * o.aL) * inv;
+        oR.q = (DLr * o.cR) * inv;
     }
 
     // ---------------------------------------------------------------- 2. wave merge
@@ -210,8 +381,8 @@ __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
             sh_blk[sl][ws][3] = oR.r; sh_blk[sl][ws][4] = oR.p; sh_blk[sl][ws][5] = oR.q;
         }
         __syncthreads();
-        if (threadIdx.x < S) {
-            const int q = threadIdx.x;
+        if (tid < S) {
+            const int q = tid;
             T rho = 0, sig = 0;
             // forward: x_{E_{w-1}} = rho - sig x_{F_w}
 #pragma unroll
@@ -247,15 +418,20 @@ __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
     // ---------------------------------------------------------------- 4. substitution
     const T xL = oL.r - oL.p * u - oL.q * v;
     const T xR = oR.r - oR.p * u - oR.q * v;
+    T *de = o.de;
     if constexpr (M > 2) {
         T xn = xR;
 #pragma unroll
         for (int j = M - 3; j >= 0; --j) {
-            xn = de[j] - al[j] * xL - ga[j] * xn;
+            T alj;
+            if constexpr (ALDS) alj = sh_al[j][tid];
+            else alj = al_r[j];
+            xn = (de[j] - alj * xL) - o.ga[j] * xn;
             de[j] = xn;
         }
     }
     if (!valid) return;
+    T *__restrict__ D = prm.d;
 #pragma unroll
     for (int j0 = 0; j0 < M; j0 += V) {
         T xv[V];
@@ -264,18 +440,32 @@ __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
             const int j = j0 + vv;
             xv[vv] = (j == 0) ? xL : ((j == M - 1) ? xR : de[(j >= 1 && j <= M - 2) ? j - 1 : 0]);
         }
-        const int64_t r0 = row0 + j0;
-        if (V > 1 && r0 + V <= n) {
-            using VT = typename Vec16<T>::type;
-            VT xs;
+        if (!BIG) {
+            const int64_t r0 = row0 + j0;
+            if (V > 1 && (fast || r0 + V <= n)) {
+                u32x4 xs;
 #pragma unroll
-            for (int vv = 0; vv < V; ++vv) reinterpret_cast<T *>(&xs)[vv] = xv[vv];
-            *reinterpret_cast<VT *>(D + base + r0) = xs;
+                for (int vv = 0; vv < V; ++vv) reinterpret_cast<T *>(&xs)[vv] = xv[vv];
+                __builtin_amdgcn_raw_buffer_store_b128(xs, rd_rsrc, vo, j0 * rowb, 0);
+            } else {
+#pragma unroll
+                for (int vv = 0; vv < V; ++vv)
+                    if (fast || r0 + vv < n) bstore<T>(xv[vv], rd_rsrc, vo, (j0 + vv) * rowb);
+            }
         } else {
+            const int64_t r0 = row0 + j0;
+            if (V > 1 && r0 + V <= n) {
+                using VT = typename Vec16<T>::type;
+                VT xs;
 #pragma unroll
-            for (int vv = 0; vv < V; ++vv) {
-                const int64_t r = r0 + vv;
-                if (r < n) D[base + r * rs] = xv[vv];
+                for (int vv = 0; vv < V; ++vv) reinterpret_cast<T *>(&xs)[vv] = xv[vv];
+                *reinterpret_cast<VT *>(D + base + r0) = xs;
+            } else {
+#pragma unroll
+                for (int vv = 0; vv < V; ++vv) {
+                    const int64_t r = r0 + vv;
+                    if (r < n) D[base + r * rs] = xv[vv];
+                }
             }
         }
     }
