@@ -1,15 +1,17 @@
 #!/usr/bin/env python3
 """Benchmark of the batched tridiagonal solve on one MI355X.
 
-Workload (the headline shape): n = 4096 rows x batch = 4096 systems, coefficients
-interleaved across the batch (element (s, i) at i*batch + s), seeded diagonally
-dominant inputs (a, c ~ U[-1,1], a_0 = c_{n-1} = 0, b = |a|+|c|+U[1,2],
-d ~ U[-1,1]).  One step = one in-place solve of the whole batch, inputs already
-resident in HBM.  Three input sets are rotated so the 256 MB MALL cannot serve
-a step from the previous one.
+Workload (the headline shape): n = 4096 rows x batch = 4096 systems, seeded
+diagonally dominant inputs (a, c ~ U[-1,1], a_0 = c_{n-1} = 0, b = |a|+|c|+U[1,2],
+d ~ U[-1,1]), in the reference's contiguous per-system layout (row i of system s
+at s*n + i) and in the interleaved layout (i*batch + s).  One step = one
+in-place solve of the whole batch, inputs already resident in HBM.  Three input
+sets are rotated so the 256 MB MALL cannot serve a step from the previous one.
 
-Prints ONE JSON line: value = unknowns/s of the FP32 solve (whole job), with the
-FP64 solve, the HBM roofline of the solve kernel and the CPU baseline alongside.
+Prints ONE JSON line: value = unknowns/s of the FP32 contiguous solve (whole job),
+with FP64 and the interleaved layout, the HBM roofline of each solve kernel
+(algorithmic bytes 5*sizeof(T) per unknown / kernel time from HIP events) and
+the CPU baseline alongside.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -31,35 +33,40 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_sets(torch, dtype, n, batch, nsets, seed0):
-    import numpy as np
+def make_sets(torch, dtype, n, batch, nsets, seed0, layout):
     import oracle
     sets = []
     for k in range(nsets):
-        a, b, c, d = oracle.gen_dd(n, batch, layout=2, seed=seed0 + k)
+        a, b, c, d = oracle.gen_dd(n, batch, layout=layout, seed=seed0 + k)
         sets.append([torch.from_numpy(v.astype(dtype)).cuda() for v in (a, b, c, d)])
         del a, b, c, d
     return sets
 
 
 def time_solver(torch, T, sets, n, batch, steps, warmup, algo, layout):
-    """Returns (ms per step wall, mean kernel ms from events)."""
+    """Times `steps` back-to-back solves through the C-ABI on torch's current stream.
+    Returns (wall ms per step, ms per launch from HIP events around the timed region)."""
+    import ctypes
     stream = torch.cuda.current_stream()
+    prec = 0 if sets[0][0].dtype == torch.float32 else 1
+    L = T.lib()
+    h = ctypes.c_void_p(stream.cuda_stream)
+    args = [(prec, algo, layout, n, batch) + tuple(ctypes.c_void_p(t.data_ptr()) for t in s) + (h,)
+            for s in sets]
     for k in range(warmup):
-        s = sets[k % len(sets)]
-        T.solve(*s, algo=algo, layout=layout, n=n, batch=batch)
+        T.check(L.tridiag_solve(*args[k % len(args)]))
     torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(steps)]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    rcs = 0
     t0 = time.perf_counter()
+    e0.record(stream)
     for k in range(steps):
-        s = sets[k % len(sets)]
-        ev[k][0].record(stream)
-        T.solve(*s, algo=algo, layout=layout, n=n, batch=batch)
-        ev[k][1].record(stream)
+        rcs |= L.tridiag_solve(*args[k % len(args)])
+    e1.record(stream)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3 / steps
-    kern = sum(e0.elapsed_time(e1) for e0, e1 in ev) / steps
+    T.check(rcs)
+    kern = e0.elapsed_time(e1) / steps
     return wall, kern
 
 
@@ -115,6 +122,42 @@ def load_traffic(name):
         return json.load(f).get(name)
 
 
+WORKLOADS = [
+    # name, numpy dtype name, layout
+    ("f32_contiguous", "float32", 0),
+    ("f64_contiguous", "float64", 0),
+    ("f32_interleaved", "float32", 2),
+    ("f64_interleaved", "float64", 2),
+]
+
+
+def run_workload(torch, T, name, dtname, layout, steps, warmup):
+    import numpy as np
+    dtype = np.dtype(dtname).type
+    n, batch = N_EQT, N_BATCH
+    log("prepare", name)
+    sets = make_sets(torch, dtype, n, batch, N_SETS, seed0=11, layout=layout)
+    log("time", name)
+    wall, kern = time_solver(torch, T, sets, n, batch, steps, warmup, -1, layout)
+    del sets
+    torch.cuda.empty_cache()
+    w = np.dtype(dtype).itemsize
+    alg_bytes = 5 * w * n * batch
+    prec = 0 if w == 4 else 1
+    pl = T.plan(prec, -1, layout, n, batch)
+    log("parity", name)
+    par = check_parity(torch, T, dtype, n, batch, -1, layout, 1e-5 if w == 4 else 1e-12)
+    achieved = alg_bytes / (kern * 1e-3) / 1e9
+    r = dict(ms_per_step=wall, kernel_ms=kern, unknowns_per_s=n * batch / (wall * 1e-3),
+             achieved_gbs=achieved, alg_bytes=alg_bytes, plan=pl, parity=par,
+             layout={0: "contiguous", 2: "interleaved"}[layout],
+             roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                       "traffic": load_traffic(name + "_4096x4096")})
+    log(name, json.dumps(r))
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,34 +170,14 @@ def main():
               "independent; sharding the batch across GPUs needs no communication).",
               file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
     import tridiagonalsolver_amd as T
     T.lib()
     torch.cuda.set_device(0)
-    n, batch = N_EQT, N_BATCH
-    unknowns = n * batch
-    results = {}
-    for name, dtype in (("f32", np.float32), ("f64", np.float64)):
-        log("prepare", name, "sets", N_SETS)
-        sets = make_sets(torch, dtype, n, batch, N_SETS, seed0=11)
-        log("time", name)
-        wall, kern = time_solver(torch, T, sets, n, batch, args.steps, args.warmup, -1, 2)
-        del sets
-        torch.cuda.empty_cache()
-        w = np.dtype(dtype).itemsize
-        alg_bytes = 5 * w * unknowns
-        pl = T.plan(0 if name == "f32" else 1, -1, 2, n, batch)
-        log("parity", name)
-        par = check_parity(torch, T, dtype, n, batch, -1, 2, 1e-5 if name == "f32" else 1e-12)
-        results[name] = dict(ms_per_step=wall, kernel_ms=kern, unknowns_per_s=unknowns / (wall * 1e-3),
-                             achieved_gbs=alg_bytes / (kern * 1e-3) / 1e9, alg_bytes=alg_bytes,
-                             plan=pl, parity=par)
-        log(name, json.dumps(results[name]))
-    main_r = results["f32"]
-    roof = {"bound": "hbm", "achieved": round(main_r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(main_r["achieved_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic("f32_interleaved_4096x4096")}
+    res = {}
+    for name, dtname, layout in WORKLOADS:
+        res[name] = run_workload(torch, T, name, dtname, layout, args.steps, args.warmup)
+    main_r = res["f32_contiguous"]
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -165,19 +188,16 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": main_r["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "batched tridiagonal solve, n=4096 x batch=4096, interleaved "
-                               "layout, FP32 (FP64 in f64)", "n": n, "batch": batch,
-                   "layout": "interleaved", "algorithm": "reg WM-pGE (auto)", "plan": main_r["plan"]},
-        "roofline": roof,
+        "config": {"workload": "batched tridiagonal solve (reg WM-pGE, auto plan), n=4096 x "
+                               "batch=4096, FP32, contiguous per-system layout (reference "
+                               "layout 0, whole-line loads); FP64 and the interleaved layout "
+                               "in 'workloads'",
+                   "n": N_EQT, "batch": N_BATCH, "layout": "contiguous", "plan": main_r["plan"]},
+        "roofline": main_r["roofline"],
         "cpu_baseline": cpu,
-        "parity": {"f32": main_r["parity"], "f64": results["f64"]["parity"]},
-        "f64": {"value": results["f64"]["unknowns_per_s"], "unit": "unknowns/s",
-                "ms_per_step": results["f64"]["ms_per_step"],
-                "roofline": {"bound": "hbm", "achieved": round(results["f64"]["achieved_gbs"], 1),
-                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": round(results["f64"]["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                             "traffic": load_traffic("f64_interleaved_4096x4096")},
-                "plan": results["f64"]["plan"]},
+        "workloads": {k: {kk: v[kk] for kk in ("unknowns_per_s", "ms_per_step", "kernel_ms",
+                                               "layout", "plan", "parity", "roofline")}
+                      for k, v in res.items()},
     }
     print(json.dumps(line), flush=True)
 

@@ -172,11 +172,13 @@ int tridiag_plan(int precision, int algo, int layout, int64_t n_eqt, int64_t n_b
  * entry `index` (rows per lane M, waves per workgroup NW, systems per workgroup
  * S, interleaved layout, 16-byte vector loads, one coefficient kept in LDS,
  * largest n_eqt, 64-bit addressing for arrays of 2 GiB or more, padded: accepts
- * n_eqt below the capacity; unpadded entries need n_eqt == capacity), and a
- * solve forced onto that entry. */
+ * n_eqt below the capacity; unpadded entries need n_eqt == capacity,
+ * nontemporal loads/stores, rows handed to lanes through LDS after whole-line
+ * loads), and a solve forced onto that entry. */
 int tridiag_tile_count(int precision);
 int tridiag_tile_info(int precision, int index, int *M, int *NW, int *S, int *interleaved,
-                      int *vec, int *lds_alpha, int64_t *cap, int *wide_addr, int *padded);
+                      int *vec, int *lds_alpha, int64_t *cap, int *wide_addr, int *padded,
+                      int *nontemporal, int *transposed);
 int tridiag_solve_tile(int precision, int index, int layout, int64_t n_eqt, int64_t n_batch,
                        void *a, void *b, void *c, void *d, void *stream);
 

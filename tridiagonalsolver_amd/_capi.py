@@ -76,7 +76,8 @@ def lib():
     L.tridiag_tile_count.argtypes = [ci]
     L.tridiag_tile_count.restype = ci
     pi = ctypes.POINTER(ci)
-    L.tridiag_tile_info.argtypes = [ci, ci, pi, pi, pi, pi, pi, pi, ctypes.POINTER(i64), pi, pi]
+    L.tridiag_tile_info.argtypes = [ci, ci, pi, pi, pi, pi, pi, pi, ctypes.POINTER(i64), pi, pi,
+                                    pi, pi]
     L.tridiag_tile_info.restype = ci
     L.tridiag_solve_tile.argtypes = [ci, ci, ci, i64, i64, vp, vp, vp, vp, vp]
     L.tridiag_solve_tile.restype = ci
@@ -109,11 +110,14 @@ def tiles(precision):
     out = []
     for i in range(L.tridiag_tile_count(precision)):
         v = [ctypes.c_int() for _ in range(6)]
-        cap, big, pad = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        cap = ctypes.c_int64()
+        big, pad, nt, xp = (ctypes.c_int() for _ in range(4))
         check(L.tridiag_tile_info(precision, i, *(ctypes.byref(x) for x in v), ctypes.byref(cap),
-                                  ctypes.byref(big), ctypes.byref(pad)))
+                                  ctypes.byref(big), ctypes.byref(pad), ctypes.byref(nt),
+                                  ctypes.byref(xp)))
         M, NW, S, il, vec, alds = (x.value for x in v)
         out.append(dict(index=i, M=M, NW=NW, S=S, interleaved=bool(il), vec=bool(vec),
                         lds_alpha=bool(alds), cap=cap.value, wide_addr=bool(big.value),
-                        padded=bool(pad.value)))
+                        padded=bool(pad.value), nontemporal=bool(nt.value),
+                        transposed=bool(xp.value)))
     return out

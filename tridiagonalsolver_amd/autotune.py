@@ -6,8 +6,9 @@ one whose answer matches the FP64 reference solution.
         --out results.json [--table tuning/tuning_gfx950.txt]
 
 The table file holds one line per bucket,
-    precision layout family n_max M NW S vec lds_alpha
-(family 0 = the default on-chip solver)
+    precision layout family n_max M NW S vec lds_alpha nt xp
+(family 0 = the default on-chip solver; nt = nontemporal loads/stores;
+xp = whole-line loads handed to lanes through LDS)
 and is read by libtridiag_amd.so at first use (TRIDIAG_TUNING_FILE overrides its
 location): for a solve of n rows the first bucket with n <= n_max whose kernel
 exists is used.
@@ -99,8 +100,10 @@ def sweep(prec, layout, n, batch, reps=20, warmup=3, nsets=3, seed=1):
         gbs = 5 * w * n * batch / (med * 1e-3) / 1e9
         rec = dict(t, ms=med, ms_min=ms[0], gbs=gbs, relerr=err)
         out.append(rec)
-        _log("prec %d layout %d n %d: M=%d NW=%d S=%d vec=%d alds=%d  %.4f ms  %.0f GB/s  err %.2e"
-             % (prec, layout, n, t["M"], t["NW"], t["S"], t["vec"], t["lds_alpha"], med, gbs, err))
+        _log("prec %d layout %d n %d: M=%d NW=%d S=%d vec=%d alds=%d nt=%d xp=%d  %.4f ms  "
+             "%.0f GB/s  err %.2e" % (prec, layout, n, t["M"], t["NW"], t["S"], t["vec"],
+                                      t["lds_alpha"], t["nontemporal"], t["transposed"], med, gbs,
+                                      err))
         for s, p in zip(sets, pristine):      # in-place solves changed d; restore
             s[3].copy_(p)
     return out
@@ -115,10 +118,12 @@ def write_table(path, rows):
     """rows: list of (prec, layout, n_max, entry) tuples."""
     os.makedirs(os.path.dirname(path), exist_ok=True)
     with open(path, "w") as f:
-        f.write("# precision layout family n_max M NW S vec lds_alpha  (written by autotune.py)\n")
+        f.write("# precision layout family n_max M NW S vec lds_alpha nt xp  "
+                "(written by autotune.py)\n")
         for prec, layout, nmax, e in sorted(rows, key=lambda r: (r[0], r[1], r[2])):
-            f.write("%d %d 0 %d %d %d %d %d %d\n" % (prec, layout, nmax, e["M"], e["NW"], e["S"],
-                                                     int(e["vec"]), int(e["lds_alpha"])))
+            f.write("%d %d 0 %d %d %d %d %d %d %d %d\n" % (
+                prec, layout, nmax, e["M"], e["NW"], e["S"], int(e["vec"]), int(e["lds_alpha"]),
+                int(e["nontemporal"]), int(e["transposed"])))
 
 
 def main(argv=None):
@@ -144,9 +149,9 @@ def main(argv=None):
                 b = best(res, 1e-5 if prec == 0 else 1e-12)
                 if b:
                     rows.append((prec, layout, n, b))
-                    _log("best prec %d layout %d n %d -> M=%d NW=%d S=%d vec=%d alds=%d %.4f ms"
-                         % (prec, layout, n, b["M"], b["NW"], b["S"], b["vec"], b["lds_alpha"],
-                            b["ms"]))
+                    _log("best prec %d layout %d n %d -> M=%d NW=%d S=%d vec=%d alds=%d nt=%d xp=%d "
+                         "%.4f ms" % (prec, layout, n, b["M"], b["NW"], b["S"], b["vec"],
+                                      b["lds_alpha"], b["nontemporal"], b["transposed"], b["ms"]))
     if args.out:
         with open(args.out, "w") as f:
             json.dump(allres, f, indent=1)

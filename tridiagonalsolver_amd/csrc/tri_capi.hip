@@ -51,13 +51,13 @@ static const char *const kNamesDouble[] = {
 static int algo_count(int prec) { return prec == TRIDIAG_SINGLE ? 18 : 14; }
 
 // ---------------------------------------------------------------- tuning table
-// Lines "precision layout family n_max M NW S vec lds_alpha" written by the
+// Lines "precision layout family n_max M NW S vec lds_alpha nt xp" written by the
 // autotuner (tridiagonalsolver_amd/autotune.py).  Read once, from
 // $TRIDIAG_TUNING_FILE or <library dir>/../tuning/tuning_gfx950.txt.
 struct TuneRow {
     int prec, layout, family;
     long long nmax;
-    int M, NW, S, vec, alds;
+    int M, NW, S, vec, alds, nt, xp;
 };
 static std::vector<TuneRow> g_tune;
 static std::once_flag g_tune_once;
@@ -82,8 +82,8 @@ static void load_tuning() {
     while (fgets(line, sizeof line, f)) {
         if (line[0] == '#') continue;
         TuneRow r;
-        if (sscanf(line, "%d %d %d %lld %d %d %d %d %d", &r.prec, &r.layout, &r.family, &r.nmax,
-                   &r.M, &r.NW, &r.S, &r.vec, &r.alds) == 9)
+        if (sscanf(line, "%d %d %d %lld %d %d %d %d %d %d %d", &r.prec, &r.layout, &r.family,
+                   &r.nmax, &r.M, &r.NW, &r.S, &r.vec, &r.alds, &r.nt, &r.xp) == 11)
             g_tune.push_back(r);
     }
     fclose(f);
@@ -110,15 +110,18 @@ static int forced_M(int algo) {
 
 template <typename T>
 static const TileEntry<T> *find_tile(int M, int NW, int S, bool il, bool vec, bool alds, bool pad,
-                                     bool big) {
+                                     bool big, bool nt, bool xp) {
     TileTable<T> tab = tile_table<T>();
+    const TileEntry<T> *any = nullptr;
     for (int i = 0; i < tab.count; ++i) {
         const TileEntry<T> &e = tab.entries[i];
         if (e.M == M && e.NW == NW && e.S == S && e.IL == il && e.VEC == vec && e.ALDS == alds &&
-            e.PAD == pad && e.BIG == big)
-            return &e;
+            e.PAD == pad && e.BIG == big) {
+            if (e.NT == nt && e.XP == xp) return &e;
+            if (!e.XP) any = &e;   // padded kernels have no streaming/transposed variant
+        }
     }
-    return nullptr;
+    return any;
 }
 
 template <typename T>
@@ -136,12 +139,13 @@ static const TileEntry<T> *choose_tile(int algo, int layout, int64_t n, bool vec
             const int nws = il ? r.NW : r.NW / r.S;
             const int64_t cap = (int64_t)(il ? 64 / r.S : 64) * nws * r.M;
             if (cap < n || cap > 2 * n) continue;   // bucket too coarse for this n
-            const TileEntry<T> *e = find_tile<T>(r.M, r.NW, r.S, il, r.vec, r.alds, cap != n, false);
+            const TileEntry<T> *e =
+                find_tile<T>(r.M, r.NW, r.S, il, r.vec, r.alds, cap != n, false, r.nt, r.xp);
             if (e) return e;
         }
     }
     const TileEntry<T> *best = nullptr;
-    long long best_key[6] = {0, 0, 0, 0, 0, 0};
+    long long best_key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = 0; i < tab.count; ++i) {
         const TileEntry<T> &e = tab.entries[i];
         if (e.IL != il || e.cap < n || e.BIG != big) continue;
@@ -150,17 +154,19 @@ static const TileEntry<T> *choose_tile(int algo, int layout, int64_t n, bool vec
         if (fM && e.M != fM) continue;
         const int nws = il ? e.NW : e.NW / e.S;
         // Register-heavy segments keep one coefficient in LDS.
-        const bool want_alds = e.M * (int)sizeof(T) >= 256;
-        long long key[6] = {
+        const bool want_alds = e.M * (int)sizeof(T) >= 256 || (e.NW >= 16 && e.M * (int)sizeof(T) >= 128);
+        long long key[8] = {
             il ? (e.S == pref_S_il<T>() ? 0 : 1) : 0,  // 32-byte row groups in IL
             (long long)e.cap,                          // least padding
-            il ? 0 : (long long)nws,                   // fewest waves per system
             (long long)std::abs(e.M - pref_M<T>(il)),  // measured sweet spot per layout
+            il ? 0 : -(long long)nws,                  // then more waves per system
             e.ALDS == want_alds ? 0 : 1,
-            e.VEC ? 0 : 1};
-        if (!best || std::lexicographical_compare(key, key + 6, best_key, best_key + 6)) {
+            e.VEC ? 0 : 1,
+            e.XP ? 0 : 1,
+            e.NT ? 0 : 1};
+        if (!best || std::lexicographical_compare(key, key + 8, best_key, best_key + 8)) {
             best = &e;
-            std::copy(key, key + 6, best_key);
+            std::copy(key, key + 8, best_key);
         }
     }
     return best;
@@ -299,7 +305,7 @@ template <typename T> static void ref_call(int algo, int layout, T *a, T *b, T *
 
 template <typename T>
 static int tile_info_impl(int index, int *M, int *NW, int *S, int *il, int *vec, int *alds,
-                          int64_t *cap, int *big, int *pad) {
+                          int64_t *cap, int *big, int *pad, int *nt, int *xp) {
     TileTable<T> tab = tile_table<T>();
     if (index < 0 || index >= tab.count) return fail(TRIDIAG_EINVAL, "tile index out of range");
     const TileEntry<T> &e = tab.entries[index];
@@ -312,6 +318,8 @@ static int tile_info_impl(int index, int *M, int *NW, int *S, int *il, int *vec,
     if (cap) *cap = e.cap;
     if (big) *big = e.BIG;
     if (pad) *pad = e.PAD;
+    if (nt) *nt = e.NT;
+    if (xp) *xp = e.XP;
     return TRIDIAG_OK;
 }
 
@@ -429,13 +437,14 @@ int tridiag_tile_count(int precision) {
 }
 
 int tridiag_tile_info(int precision, int index, int *M, int *NW, int *S, int *interleaved,
-                      int *vec, int *lds_alpha, int64_t *cap, int *wide_addr, int *padded) {
+                      int *vec, int *lds_alpha, int64_t *cap, int *wide_addr, int *padded,
+                      int *nontemporal, int *transposed) {
     if (precision == TRIDIAG_SINGLE)
         return tile_info_impl<float>(index, M, NW, S, interleaved, vec, lds_alpha, cap, wide_addr,
-                                     padded);
+                                     padded, nontemporal, transposed);
     if (precision == TRIDIAG_DOUBLE)
         return tile_info_impl<double>(index, M, NW, S, interleaved, vec, lds_alpha, cap, wide_addr,
-                                      padded);
+                                      padded, nontemporal, transposed);
     return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
 }
 

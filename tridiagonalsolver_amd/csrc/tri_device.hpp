@@ -33,6 +33,15 @@ __device__ __forceinline__ double lane_read(double v, int src) {
     return *reinterpret_cast<double *>(&p);
 }
 
+// Orders LDS traffic between the lanes of one wave: every LDS write issued
+// before it is visible to every lane's LDS reads after it, and the compiler may
+// not move LDS accesses across it.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T> struct Vec16;
 template <> struct Vec16<float> { using type = float4; static constexpr int N = 4; };
 template <> struct Vec16<double> { using type = double2; static constexpr int N = 2; };

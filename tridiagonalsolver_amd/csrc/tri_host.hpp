@@ -14,17 +14,19 @@ enum Layout { kContiguous = 0, kPacked = 1, kInterleaved = 2 };
 // One compiled tile kernel.
 template <typename T> struct TileEntry {
     int M, NW, S;
-    bool IL, VEC, ALDS, BIG, PAD;
+    bool IL, VEC, ALDS, BIG, PAD, NT, XP;
     int cap;  // rows per system it can hold
     hipError_t (*launch)(const TileParams<T> &, hipStream_t);
 };
 
-template <typename T, int M, int NW, int S, bool IL, bool VEC, bool ALDS, bool BIG, bool PAD>
+template <typename T, int M, int NW, int S, bool IL, bool VEC, bool ALDS, bool BIG, bool PAD,
+          bool NT, bool XP>
 hipError_t launch_tile(const TileParams<T> &p, hipStream_t st) {
     const int64_t tiles = (p.batch + S - 1) / S;
     if (tiles <= 0) return hipSuccess;
     if (tiles > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((tile_wm_kernel<T, M, NW, S, IL, VEC, ALDS, BIG, PAD>), dim3((unsigned)tiles),
+    hipLaunchKernelGGL((tile_wm_kernel<T, M, NW, S, IL, VEC, ALDS, BIG, PAD, NT, XP>),
+                       dim3((unsigned)tiles),
                        dim3(64 * NW), 0, st, p);
     return hipGetLastError();
 }

@@ -62,7 +62,8 @@ template <typename T> struct TileParams {
     int32_t buf_bytes;
 };
 
-template <typename T, int M, int NW, int S, bool IL, bool VEC, bool ALDS, bool BIG, bool PAD>
+template <typename T, int M, int NW, int S, bool IL, bool VEC, bool ALDS, bool BIG, bool PAD,
+          bool NT, bool XP>
 struct TileShape {
     static_assert(M >= 2, "a lane segment needs at least two rows");
     static constexpr int M_ = M;
@@ -89,19 +90,25 @@ struct TileShape {
 // array spans < 2^31 bytes.  Buffer loads address a row as one per-lane VGPR
 // offset plus a wave-uniform SGPR row offset, so in-flight loads cost no
 // address registers.
-template <typename T> __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+// Cache-policy operand of the buffer instructions: 2 = nontemporal (nt).
+template <bool NT> constexpr int cpol() { return NT ? 2 : 0; }
+
+template <typename T, bool NT = false>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, int vo, int so) {
     if constexpr (sizeof(T) == 4)
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, cpol<NT>()));
     else
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, cpol<NT>()));
 }
-template <typename T> __device__ __forceinline__ void bstore(T x, __amdgpu_buffer_rsrc_t r, int vo,
-                                                            int so) {
+template <typename T, bool NT = false>
+__device__ __forceinline__ void bstore(T x, __amdgpu_buffer_rsrc_t r, int vo, int so) {
     if constexpr (sizeof(T) == 4)
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, vo, so, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, vo, so,
+                                              cpol<NT>());
     else
         __builtin_amdgcn_raw_buffer_store_b64(
-            __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, x), r, vo, so, 0);
+            __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, x), r, vo, so,
+            cpol<NT>());
 }
 using u32x4 = __attribute__((ext_vector_type(4))) unsigned;
 
@@ -110,7 +117,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, int32
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
 }
 
-template <typename T, int CH, int V> struct FastLoader {
+template <typename T, int CH, int V, bool NT> struct FastLoader {
     __amdgpu_buffer_rsrc_t ra, rb, rc, rd;
     int vo;       // lane byte offset of its first row
     int rowb;     // bytes between consecutive rows of one system
@@ -119,19 +126,19 @@ template <typename T, int CH, int V> struct FastLoader {
 #pragma unroll
             for (int v = 0; v < CH; ++v) {
                 const int so = (j0 + v) * rowb;
-                q[0][v] = bload<T>(ra, vo, so);
-                q[1][v] = bload<T>(rb, vo, so);
-                q[2][v] = bload<T>(rc, vo, so);
-                q[3][v] = bload<T>(rd, vo, so);
+                q[0][v] = bload<T, NT>(ra, vo, so);
+                q[1][v] = bload<T, NT>(rb, vo, so);
+                q[2][v] = bload<T, NT>(rc, vo, so);
+                q[3][v] = bload<T, NT>(rd, vo, so);
             }
         } else {
 #pragma unroll
             for (int v0 = 0; v0 < CH; v0 += V) {
                 const int so = (j0 + v0) * rowb;
-                u32x4 xa = __builtin_amdgcn_raw_buffer_load_b128(ra, vo, so, 0);
-                u32x4 xb = __builtin_amdgcn_raw_buffer_load_b128(rb, vo, so, 0);
-                u32x4 xc = __builtin_amdgcn_raw_buffer_load_b128(rc, vo, so, 0);
-                u32x4 xd = __builtin_amdgcn_raw_buffer_load_b128(rd, vo, so, 0);
+                u32x4 xa = __builtin_amdgcn_raw_buffer_load_b128(ra, vo, so, cpol<NT>());
+                u32x4 xb = __builtin_amdgcn_raw_buffer_load_b128(rb, vo, so, cpol<NT>());
+                u32x4 xc = __builtin_amdgcn_raw_buffer_load_b128(rc, vo, so, cpol<NT>());
+                u32x4 xd = __builtin_amdgcn_raw_buffer_load_b128(rd, vo, so, cpol<NT>());
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
                     q[0][v0 + v] = reinterpret_cast<const T *>(&xa)[v];
@@ -182,6 +189,17 @@ template <typename T, int CH, int V> struct PredLoader {
                 }
             }
         }
+    }
+};
+
+// Rows already staged in registers (transposed wave-block loads).
+template <typename T, int CH, int M> struct RegLoader {
+    const T (*rows)[M];
+    __device__ __forceinline__ void load(T (&q)[4][CH], int j0) const {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int v = 0; v < CH; ++v) q[k][v] = rows[k][j0 + v];
     }
 };
 
@@ -284,9 +302,17 @@ __device__ __forceinline__ void lane_sweep(SweepOut<T, Sh::NI> &o, const Loader 
 // PAD = false: the system fills the tile exactly (n == CAP), loads are not
 // predicated.  PAD = true: n < CAP, rows past the end are identity rows.
 // BIG: 64-bit addressing for arrays of 2 GiB or more (always predicated).
-template <typename T, int M, int NW, int S, bool IL, bool VEC, bool ALDS, bool BIG, bool PAD>
+// NT: nontemporal (streaming) cache policy on the coefficient loads and the x
+// stores; the data is touched once per solve.
+// XP (contiguous layout, exact tiles): a wave reads its 64*M-row block of each
+// array with fully coalesced 16-byte loads and hands every lane its own rows
+// through LDS; x goes back the same way.  Every load instruction then covers
+// whole cache lines, which is what the nontemporal policy needs.
+template <typename T, int M, int NW, int S, bool IL, bool VEC, bool ALDS, bool BIG, bool PAD,
+          bool NT, bool XP>
 __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
-    using Sh = TileShape<T, M, NW, S, IL, VEC, ALDS, BIG, PAD>;
+    using Sh = TileShape<T, M, NW, S, IL, VEC, ALDS, BIG, PAD, NT, XP>;
+    static_assert(!XP || (!IL && VEC && !PAD && !BIG), "transposed loads need exact contiguous tiles");
     constexpr int SW = Sh::SW, NWS = Sh::NWS, LS = Sh::LS, V = Sh::V, CH = Sh::CH;
     constexpr int LANES = 64 * NW;
     const int tid = threadIdx.x;
@@ -309,6 +335,8 @@ __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
     // Interior coefficients; al lives in LDS when ALDS (it is only read back in
     // the substitution), laid out [row][lane] so a wave's accesses are contiguous.
     __shared__ T sh_al[ALDS ? Sh::NI : 1][ALDS ? LANES : 1];
+    __shared__ u32x4 xbuf[XP ? NW : 1][XP ? 64 * M * (int)sizeof(T) / 16 : 1];
+    (void)xbuf;
     T al_r[ALDS ? 1 : Sh::NI];
     (void)al_r;
     (void)sh_al;
@@ -328,8 +356,37 @@ __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
     } else {
         __amdgpu_buffer_rsrc_t ra = make_rsrc(prm.a, nb), rb = make_rsrc(prm.b, nb),
                                rc = make_rsrc(prm.c, nb);
-        if constexpr (fast) {
-            FastLoader<T, CH, V> ld{ra, rb, rc, rd_rsrc, vo, rowb};
+        if constexpr (XP) {
+            // wave block: 64 segments x M rows = 64*M contiguous elements per array
+            constexpr int NQ = 64 * M * (int)sizeof(T) / 1024;   // 16-byte loads per lane
+            constexpr int PL = M * (int)sizeof(T) / 16;          // 16-byte pieces per lane row set
+            const int wvo = (int)((base + (int64_t)ws * 64 * M) * (int64_t)sizeof(T)) + lane * 16;
+            u32x4 g[4][NQ];
+            const __amdgpu_buffer_rsrc_t rr[4] = {ra, rb, rc, rd_rsrc};
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+                    g[k][q] = __builtin_amdgcn_raw_buffer_load_b128(rr[k], wvo, q * 1024, cpol<NT>());
+            T rows[4][M];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (k > 0) wave_lds_sync();       // previous array fully read by all lanes
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) xbuf[wave][q * 64 + lane] = g[k][q];
+                wave_lds_sync();                  // block written before lanes read their rows
+#pragma unroll
+                for (int i = 0; i < PL; ++i) {
+                    u32x4 t = xbuf[wave][lane * PL + i];
+#pragma unroll
+                    for (int v = 0; v < 16 / (int)sizeof(T); ++v)
+                        rows[k][i * (16 / (int)sizeof(T)) + v] = reinterpret_cast<const T *>(&t)[v];
+                }
+            }
+            RegLoader<T, CH, M> ld{rows};
+            lane_sweep<T, Sh>(o, ld, row0, n, al_store);
+        } else if constexpr (fast) {
+            FastLoader<T, CH, V, NT> ld{ra, rb, rc, rd_rsrc, vo, rowb};
             lane_sweep<T, Sh>(o, ld, row0, n, al_store);
         } else {
             PredLoader<T, CH, V> ld{ra, rb, rc, rd_rsrc, vo, rowb, row0, n};
@@ -431,6 +488,30 @@ __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
         }
     }
     if (!valid) return;
+    if constexpr (XP) {
+        constexpr int NQ = 64 * M * (int)sizeof(T) / 1024;
+        constexpr int PL = M * (int)sizeof(T) / 16;
+        constexpr int VV = 16 / (int)sizeof(T);
+        wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < PL; ++i) {
+            u32x4 t;
+#pragma unroll
+            for (int v = 0; v < VV; ++v) {
+                const int j = i * VV + v;
+                reinterpret_cast<T *>(&t)[v] =
+                    (j == 0) ? xL : ((j == M - 1) ? xR : de[(j >= 1 && j <= M - 2) ? j - 1 : 0]);
+            }
+            xbuf[wave][lane * PL + i] = t;
+        }
+        wave_lds_sync();                          // all rows written before the block is read
+        const int wvo = (int)((base + (int64_t)ws * 64 * M) * (int64_t)sizeof(T)) + lane * 16;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            __builtin_amdgcn_raw_buffer_store_b128(xbuf[wave][q * 64 + lane], rd_rsrc, wvo, q * 1024,
+                                                   cpol<NT>());
+        return;
+    }
     T *__restrict__ D = prm.d;
 #pragma unroll
     for (int j0 = 0; j0 < M; j0 += V) {
@@ -446,11 +527,12 @@ __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
                 u32x4 xs;
 #pragma unroll
                 for (int vv = 0; vv < V; ++vv) reinterpret_cast<T *>(&xs)[vv] = xv[vv];
-                __builtin_amdgcn_raw_buffer_store_b128(xs, rd_rsrc, vo, j0 * rowb, 0);
+                __builtin_amdgcn_raw_buffer_store_b128(xs, rd_rsrc, vo, j0 * rowb, cpol<NT>());
             } else {
 #pragma unroll
                 for (int vv = 0; vv < V; ++vv)
-                    if (fast || r0 + vv < n) bstore<T>(xv[vv], rd_rsrc, vo, (j0 + vv) * rowb);
+                    if (fast || r0 + vv < n)
+                        bstore<T, NT>(xv[vv], rd_rsrc, vo, (j0 + vv) * rowb);
             }
         } else {
             const int64_t r0 = row0 + j0;
