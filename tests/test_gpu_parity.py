@@ -63,6 +63,20 @@ def test_auto_interleaved(prec, n, batch):
     _solve_and_check(prec, -1, 2, n, batch, seed=7 * n + batch)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("algo", TILE_ALGOS)
+@pytest.mark.parametrize("n", [5, 64, 1000, 4096])
+def test_tile_algos_interleaved(prec, algo, n):
+    _solve_and_check(prec, algo, 2, n, 40, seed=3000 * algo + n)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("algo", [2, 3, 4, 5, 6, 7, 8, 10])
+def test_interface_families_low_margin(prec, algo):
+    """Every interface solver on the least dominant distribution (margin U[0.05, 0.2])."""
+    _solve_and_check(prec, algo, 0, 2048, 16, seed=77 + algo, blo=0.05, bhi=0.2)
+
+
 @pytest.mark.parametrize("algo", [14, 15, 16, 17])
 @pytest.mark.parametrize("n", [64, 1024, 2000])
 def test_bplg_packed_layout(algo, n):

@@ -91,6 +91,39 @@ static void load_tuning() {
                      [](const TuneRow &x, const TuneRow &y) { return x.nmax < y.nmax; });
 }
 
+int max_lds_per_block() {
+    static int v = [] {
+        int dev = 0, x = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 65536;
+        if (hipDeviceGetAttribute(&x, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            return 65536;
+        return x;
+    }();
+    return v;
+}
+
+// Interface solver and exchange mechanism behind each algorithm id.
+struct AlgoSpec {
+    int iface;     // Iface
+    int xchg_lds;  // Wang-Mou exchanges through LDS
+    int lds_rows;  // prefer kernels that stage per-row coefficients in LDS
+};
+static AlgoSpec algo_spec(int algo) {
+    switch (algo) {
+    case kWM5Shmem: return {kIfaceWM, 1, 1};
+    case kWM4Shmem: return {kIfaceWMHalf, 1, 1};
+    case kWM4Reg: return {kIfaceWMHalf, 0, 0};
+    case kPCR5Shmem: return {kIfacePCR, 0, 1};
+    case kPCR5Reg: return {kIfacePCR, 0, 0};
+    case kPCR4Shmem: return {kIfacePCRThomas, 0, 1};
+    case kPCR4Reg: return {kIfacePCRThomas, 0, 0};
+    case kCR: case kCRReg4: case kCRReg8: case kCRReg16: return {kIfaceCR, 0, 0};
+    case kBPLGCR: return {kIfaceCR, 0, 0};
+    case kBPLGLF: case kBPLGPCR: return {kIfacePCR, 0, 0};
+    default: return {kIfaceWM, 0, 0};   // reg WM-pGE, BPLG WM, auto
+    }
+}
+
 // ---------------------------------------------------------------- planning
 template <typename T> static int pref_M(bool il) {
     if (il) return sizeof(T) == 4 ? 32 : 16;
@@ -106,6 +139,17 @@ static int forced_M(int algo) {
     case kCRReg16: return 16;
     default: return 0;
     }
+}
+
+// LDS one workgroup of `e` needs with the interface solver of `algo`.
+template <typename T> static long lds_need(const TileEntry<T> &e, int algo) {
+    const long w = sizeof(T), lanes = 64L * e.NW;
+    const long nws = e.IL ? e.NW : e.NW / e.S;
+    long st = (e.ALDS ? (long)(e.M - 2) * lanes * w : 0) + (e.XP ? lanes * e.M * w : 0) +
+              (long)e.S * 2 * nws * 8 * w;
+    const AlgoSpec sp = algo_spec(algo);
+    long dyn = sp.iface >= kIfacePCR ? 8 * lanes * w : (sp.xchg_lds ? 6 * lanes * w : 0);
+    return st + dyn;
 }
 
 template <typename T>
@@ -129,7 +173,9 @@ static const TileEntry<T> *choose_tile(int algo, int layout, int64_t n, bool vec
                                        bool big = false) {
     TileTable<T> tab = tile_table<T>();
     const bool il = layout == kInterleaved;
-    const int fM = forced_M(algo);
+    // register-packed CR fixes R rows per lane for the reference's per-system layout;
+    // interleaved batches use the tuned shape with the same CR interface
+    const int fM = il ? 0 : forced_M(algo);
     if (!fM && !big) {
         std::call_once(g_tune_once, load_tuning);
         const int prec = sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE;
@@ -139,9 +185,12 @@ static const TileEntry<T> *choose_tile(int algo, int layout, int64_t n, bool vec
             const int nws = il ? r.NW : r.NW / r.S;
             const int64_t cap = (int64_t)(il ? 64 / r.S : 64) * nws * r.M;
             if (cap < n || cap > 2 * n) continue;   // bucket too coarse for this n
-            const TileEntry<T> *e =
-                find_tile<T>(r.M, r.NW, r.S, il, r.vec, r.alds, cap != n, false, r.nt, r.xp);
-            if (e) return e;
+            const TileEntry<T> *e = nullptr;
+            if (algo_spec(algo).lds_rows)      // LDS-staged variant of the tuned shape, if built
+                e = find_tile<T>(r.M, r.NW, r.S, il, r.vec, true, cap != n, false, r.nt, false);
+            if (e && lds_need(*e, algo) > max_lds_per_block()) e = nullptr;
+            if (!e) e = find_tile<T>(r.M, r.NW, r.S, il, r.vec, r.alds, cap != n, false, r.nt, r.xp);
+            if (e && lds_need(*e, algo) <= max_lds_per_block()) return e;
         }
     }
     const TileEntry<T> *best = nullptr;
@@ -152,9 +201,12 @@ static const TileEntry<T> *choose_tile(int algo, int layout, int64_t n, bool vec
         if (!e.BIG && e.PAD != (e.cap != n)) continue;   // exact tiles take the unpredicated kernel
         if (e.VEC && !vec_ok) continue;
         if (fM && e.M != fM) continue;
+        if (lds_need(e, algo) > max_lds_per_block()) continue;
         const int nws = il ? e.NW : e.NW / e.S;
-        // Register-heavy segments keep one coefficient in LDS.
-        const bool want_alds = e.M * (int)sizeof(T) >= 256 || (e.NW >= 16 && e.M * (int)sizeof(T) >= 128);
+        // Register-heavy segments keep one coefficient in LDS; so do the LDS-staged
+        // ("shmem") algorithms where the shape has such a kernel.
+        const bool want_alds = algo_spec(algo).lds_rows || e.M * (int)sizeof(T) >= 256 ||
+                               (e.NW >= 16 && e.M * (int)sizeof(T) >= 128);
         long long key[8] = {
             il ? (e.S == pref_S_il<T>() ? 0 : 1) : 0,  // 32-byte row groups in IL
             (long long)e.cap,                          // least padding
@@ -239,8 +291,13 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
                     "n_eqt=" + std::to_string(n) + " exceeds the on-chip capacity of algorithm " +
                         std::to_string(algo) + " in layout " + std::to_string(layout));
     }
-    TileParams<T> p{a, b, c, d, n, batch, sstride, rstride, bb};
+    const AlgoSpec sp = algo_spec(algo);
+    TileParams<T> p{a, b, c, d, n, batch, sstride, rstride, bb, sp.iface, sp.xchg_lds};
     hipError_t err = e->launch(p, st);
+    if (err == hipErrorInvalidConfiguration)
+        return fail(TRIDIAG_EUNSUPPORTED, "interface solver of algorithm " + std::to_string(algo) +
+                                              " needs more LDS than the kernel for n_eqt=" +
+                                              std::to_string(n) + " leaves");
     return err == hipSuccess ? TRIDIAG_OK : hip_fail(err, "tile launch");
 }
 
@@ -349,7 +406,7 @@ static int solve_tile_impl(int index, int layout, int64_t n, int64_t batch, T *a
         return fail(TRIDIAG_EINVAL, "vector tile needs 16-byte aligned rows");
     const int32_t bb = buffer_extent<T>(n, batch, sstride, layout);
     if (bb == 0 && !e.BIG) return fail(TRIDIAG_EUNSUPPORTED, "arrays of 2 GiB or more need a BIG tile");
-    TileParams<T> p{a, b, c, d, n, batch, sstride, rstride, bb};
+    TileParams<T> p{a, b, c, d, n, batch, sstride, rstride, bb, kIfaceWM, 0};
     hipError_t err = e.launch(p, st);
     return err == hipSuccess ? TRIDIAG_OK : hip_fail(err, "tile launch");
 }

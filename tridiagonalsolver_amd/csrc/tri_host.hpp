@@ -19,15 +19,35 @@ template <typename T> struct TileEntry {
     hipError_t (*launch)(const TileParams<T> &, hipStream_t);
 };
 
+// Largest LDS allocation of one workgroup on the current device.
+int max_lds_per_block();
+
 template <typename T, int M, int NW, int S, bool IL, bool VEC, bool ALDS, bool BIG, bool PAD,
           bool NT, bool XP>
 hipError_t launch_tile(const TileParams<T> &p, hipStream_t st) {
     const int64_t tiles = (p.batch + S - 1) / S;
     if (tiles <= 0) return hipSuccess;
     if (tiles > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((tile_wm_kernel<T, M, NW, S, IL, VEC, ALDS, BIG, PAD, NT, XP>),
-                       dim3((unsigned)tiles),
-                       dim3(64 * NW), 0, st, p);
+    auto kern = tile_wm_kernel<T, M, NW, S, IL, VEC, ALDS, BIG, PAD, NT, XP>;
+    // LDS for the interface solvers: 2 rows x 4 coefficients per lane (PCR/CR), or
+    // 6 outer-form values per lane (Wang-Mou merges exchanged through LDS).
+    size_t dyn = 0;
+    if (p.iface >= kIfacePCR) dyn = (size_t)iface_lds_bytes<T>(64 * NW);
+    else if (p.xchg_lds) dyn = (size_t)6 * 64 * NW * sizeof(T);
+    if (dyn) {
+        static size_t granted = 0;      // per instantiation
+        if (dyn > granted) {
+            hipFuncAttributes fa;
+            hipError_t e = hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(kern));
+            if (e != hipSuccess) return e;
+            if (fa.sharedSizeBytes + dyn > (size_t)max_lds_per_block()) return hipErrorInvalidConfiguration;
+            e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+            if (e != hipSuccess) return e;
+            granted = dyn;
+        }
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(64 * NW), dyn, st, p);
     return hipGetLastError();
 }
 

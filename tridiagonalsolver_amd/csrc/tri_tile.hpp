@@ -35,6 +35,7 @@
 //        a system is owned by NWS consecutive waves, lane = segment.
 #pragma once
 #include "tri_device.hpp"
+#include "tri_iface.hpp"
 
 namespace tri {
 
@@ -60,6 +61,8 @@ template <typename T> struct TileParams {
     // Byte extent of each coefficient array when it is below 2^31 (buffer
     // addressing with 32-bit offsets is allowed), else 0.
     int32_t buf_bytes;
+    int32_t iface;     // interface solver (Iface)
+    int32_t xchg_lds;  // Wang-Mou exchanges through LDS (1) or ds_bpermute (0)
 };
 
 template <typename T, int M, int NW, int S, bool IL, bool VEC, bool ALDS, bool BIG, bool PAD,
@@ -337,6 +340,7 @@ __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
     __shared__ T sh_al[ALDS ? Sh::NI : 1][ALDS ? LANES : 1];
     __shared__ u32x4 xbuf[XP ? NW : 1][XP ? 64 * M * (int)sizeof(T) / 16 : 1];
     (void)xbuf;
+    extern __shared__ __attribute__((aligned(16))) char tri_dyn_lds[];
     T al_r[ALDS ? 1 : Sh::NI];
     (void)al_r;
     (void)sh_al;
@@ -398,83 +402,129 @@ __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
     //   L: aL u + DLr x_L + ELr x_R = FLr ;  R: ARr x_L + DRr x_R + cR v = FRr
     const T DLr = o.bL - o.cL * o.SA, ELr = o.cL * o.PI, FLr = o.dL - o.cL * o.SD;
     const T ARr = -(o.aR * o.alp), DRr = o.bR - o.aR * o.g, FRr = o.dR - o.aR * o.del;
-    Outer<T> oL, oR;
-    {
-        T inv = rcp(DLr * DRr - ELr * ARr);
-        oL.r = (DRr * FLr - ELr * FRr) * inv;
-        oL.p = (DRr * o.aL) * inv;
-        oL.q = -(ELr * o.cR) * inv;
-        oR.r = (DLr * FRr - ARr * FLr) * inv;
-        oR.p = -(ARr * o.aL) * inv;
-        oR.q = (DLr * o.cR) * inv;
-    }
 
-    // ---------------------------------------------------------------- 2. wave merge
+    // ---------------------------------------------------------------- 2-3. interface
+    T xL, xR;
+    const int seg = ws * SW + kw;            // segment index inside the system
+    if (prm.iface >= kIfacePCR) {
+        // PCR / PCR + Thomas / CR on the 2P interface rows, through LDS
+        IfaceLds<T> IL_ = iface_lds<T>(tri_dyn_lds, S, 2 * Sh::P);
+        Row4<T> rl{o.aL, DLr, ELr, FLr}, rr{ARr, DRr, o.cR, FRr};
+        if (prm.iface == kIfacePCR || prm.iface == kIfacePCRThomas) {
+            const int lim = (prm.iface == kIfacePCR) ? Sh::P : SW;
+            pcr_interface(IL_, sl, seg, Sh::P, lim, rl, rr);
+            if (lim == Sh::P) {
+                xL = rl.d * rcp(rl.b);
+                xR = rr.d * rcp(rr.b);
+            } else {
+                IL_.put(sl, 2 * seg, rl);
+                IL_.put(sl, 2 * seg + 1, rr);
+                __syncthreads();
+                strided_thomas(IL_, S, Sh::P, SW, tid);
+                xL = IL_.d[sl * 2 * Sh::P + 2 * seg];
+                xR = IL_.d[sl * 2 * Sh::P + 2 * seg + 1];
+            }
+        } else {
+            IL_.put(sl, 2 * seg, rl);
+            IL_.put(sl, 2 * seg + 1, rr);
+            __syncthreads();
+            cr_solve(IL_, S, tid, LANES);
+            xL = IL_.d[sl * 2 * Sh::P + 2 * seg];
+            xR = IL_.d[sl * 2 * Sh::P + 2 * seg + 1];
+        }
+    } else {
+        // Wang-Mou: outer form of the segment, merges across lanes, pGE chain across blocks
+        Outer<T> oL, oR;
+        {
+            T inv = rcp(DLr * DRr - ELr * ARr);
+            oL.r = (DRr * FLr - ELr * FRr) * inv;
+            oL.p = (DRr * o.aL) * inv;
+            oL.q = -(ELr * o.cR) * inv;
+            oR.r = (DLr * FRr - ARr * FLr) * inv;
+            oR.p = -(ARr * o.aL) * inv;
+            oR.q = (DLr * o.cR) * inv;
+        }
+        const int half = prm.iface == kIfaceWMHalf ? 1 : 0;
+        const int BS = SW >> half;                   // segments per merged block
+        const bool lds_x = prm.xchg_lds != 0;        // exchange through LDS instead of bpermute
+        T *xs = reinterpret_cast<T *>(tri_dyn_lds) + (size_t)(tid & ~63) * 6;
 #pragma unroll
-    for (int t = 0; (1 << t) < SW; ++t) {
-        const int h = 1 << t;
-        const int blk = kw >> t;
-        const int pb = (blk & ~1) << t;              // first segment of the merged pair
-        const int srcA = sl % LS + LS * (pb + h - 1); // lane with A's last row
-        const int srcB = sl % LS + LS * (pb + h);     // lane with B's first row
-        T rA = lane_read(oR.r, srcA), pA = lane_read(oR.p, srcA), qA = lane_read(oR.q, srcA);
-        T rB = lane_read(oL.r, srcB), pB = lane_read(oL.p, srcB), qB = lane_read(oL.q, srcB);
-        T RB, PB, QB, RA, PA, QA;
-        wm_pair_coeffs(rA, pA, qA, rB, pB, qB, RB, PB, QB, RA, PA, QA);
-        const bool inA = (blk & 1) == 0;
-        wm_merge_row(oL, inA, RB, PB, QB, RA, PA, QA);
-        wm_merge_row(oR, inA, RB, PB, QB, RA, PA, QA);
-    }
+        for (int t = 0; (1 << t) < SW; ++t) {
+            if ((1 << t) >= BS) break;
+            const int h = 1 << t;
+            const int blk = kw >> t;
+            const int pb = (blk & ~1) << t;              // first segment of the merged pair
+            const int srcA = sl % LS + LS * (pb + h - 1); // lane with A's last row
+            const int srcB = sl % LS + LS * (pb + h);     // lane with B's first row
+            T rA, pA, qA, rB, pB, qB;
+            if (lds_x) {
+                xs[lane * 6 + 0] = oL.r; xs[lane * 6 + 1] = oL.p; xs[lane * 6 + 2] = oL.q;
+                xs[lane * 6 + 3] = oR.r; xs[lane * 6 + 4] = oR.p; xs[lane * 6 + 5] = oR.q;
+                wave_lds_sync();
+                rA = xs[srcA * 6 + 3]; pA = xs[srcA * 6 + 4]; qA = xs[srcA * 6 + 5];
+                rB = xs[srcB * 6 + 0]; pB = xs[srcB * 6 + 1]; qB = xs[srcB * 6 + 2];
+                wave_lds_sync();
+            } else {
+                rA = lane_read(oR.r, srcA); pA = lane_read(oR.p, srcA); qA = lane_read(oR.q, srcA);
+                rB = lane_read(oL.r, srcB); pB = lane_read(oL.p, srcB); qB = lane_read(oL.q, srcB);
+            }
+            T RB, PB, QB, RA, PA, QA;
+            wm_pair_coeffs(rA, pA, qA, rB, pB, qB, RB, PB, QB, RA, PA, QA);
+            const bool inA = (blk & 1) == 0;
+            wm_merge_row(oL, inA, RB, PB, QB, RA, PA, QA);
+            wm_merge_row(oR, inA, RB, PB, QB, RA, PA, QA);
+        }
 
-    // ---------------------------------------------------------------- 3. WG chain
-    T u = 0, v = 0;
-    if constexpr (NWS > 1) {
-        __shared__ T sh_blk[S][NWS][6];
-        __shared__ T sh_uv[S][NWS][2];
-        if (kw == 0) {
-            sh_blk[sl][ws][0] = oL.r; sh_blk[sl][ws][1] = oL.p; sh_blk[sl][ws][2] = oL.q;
-        }
-        if (kw == SW - 1) {
-            sh_blk[sl][ws][3] = oR.r; sh_blk[sl][ws][4] = oR.p; sh_blk[sl][ws][5] = oR.q;
-        }
-        __syncthreads();
-        if (tid < S) {
-            const int q = tid;
-            T rho = 0, sig = 0;
-            // forward: x_{E_{w-1}} = rho - sig x_{F_w}
-#pragma unroll
-            for (int w = 0; w < NWS; ++w) {
-                T rF = sh_blk[q][w][0], pF = sh_blk[q][w][1], qF = sh_blk[q][w][2];
-                T rE = sh_blk[q][w][3], pE = sh_blk[q][w][4], qE = sh_blk[q][w][5];
-                T inv = rcp(T(1) - pF * sig);
-                T rhoF = (rF - pF * rho) * inv;
-                T sigF = qF * inv;
-                T rhoE = rE - pE * (rho - sig * rhoF);
-                T sigE = qE + pE * sig * sigF;
-                sh_blk[q][w][0] = rhoF; sh_blk[q][w][1] = sigF;
-                sh_blk[q][w][2] = rhoE; sh_blk[q][w][3] = sigE;
-                rho = rhoE; sig = sigE;
+        // pGE chain over the NB = NWS << half merged blocks of each system
+        T u = 0, v = 0;
+        const int NB = NWS << half;
+        if (NB > 1) {
+            __shared__ T sh_blk[S][2 * NWS][6];
+            __shared__ T sh_uv[S][2 * NWS][2];
+            const int bi = ws * (1 << half) + kw / BS;
+            if (kw % BS == 0) {
+                sh_blk[sl][bi][0] = oL.r; sh_blk[sl][bi][1] = oL.p; sh_blk[sl][bi][2] = oL.q;
             }
-            // backward: x_{F_w} and x_{E_w} from x_{F_{w+1}}
-            T xFn = 0;
-            sh_uv[q][0][0] = 0;
-#pragma unroll
-            for (int w = NWS - 1; w >= 0; --w) {
-                T xE = sh_blk[q][w][2] - sh_blk[q][w][3] * xFn;
-                T xF = sh_blk[q][w][0] - sh_blk[q][w][1] * xFn;
-                sh_uv[q][w][1] = xFn;
-                if (w + 1 < NWS) sh_uv[q][w + 1][0] = xE;
-                xFn = xF;
+            if (kw % BS == BS - 1) {
+                sh_blk[sl][bi][3] = oR.r; sh_blk[sl][bi][4] = oR.p; sh_blk[sl][bi][5] = oR.q;
             }
+            __syncthreads();
+            if (tid < S) {
+                const int q = tid;
+                T rho = 0, sig = 0;
+                // forward: x_{E_{w-1}} = rho - sig x_{F_w}
+                for (int w = 0; w < NB; ++w) {
+                    T rF = sh_blk[q][w][0], pF = sh_blk[q][w][1], qF = sh_blk[q][w][2];
+                    T rE = sh_blk[q][w][3], pE = sh_blk[q][w][4], qE = sh_blk[q][w][5];
+                    T inv = rcp(T(1) - pF * sig);
+                    T rhoF = (rF - pF * rho) * inv;
+                    T sigF = qF * inv;
+                    T rhoE = rE - pE * (rho - sig * rhoF);
+                    T sigE = qE + pE * sig * sigF;
+                    sh_blk[q][w][0] = rhoF; sh_blk[q][w][1] = sigF;
+                    sh_blk[q][w][2] = rhoE; sh_blk[q][w][3] = sigE;
+                    rho = rhoE; sig = sigE;
+                }
+                // backward: x_{F_w} and x_{E_w} from x_{F_{w+1}}
+                T xFn = 0;
+                sh_uv[q][0][0] = 0;
+                for (int w = NB - 1; w >= 0; --w) {
+                    T xE = sh_blk[q][w][2] - sh_blk[q][w][3] * xFn;
+                    T xF = sh_blk[q][w][0] - sh_blk[q][w][1] * xFn;
+                    sh_uv[q][w][1] = xFn;
+                    if (w + 1 < NB) sh_uv[q][w + 1][0] = xE;
+                    xFn = xF;
+                }
+            }
+            __syncthreads();
+            u = sh_uv[sl][bi][0];
+            v = sh_uv[sl][bi][1];
         }
-        __syncthreads();
-        u = sh_uv[sl][ws][0];
-        v = sh_uv[sl][ws][1];
+        xL = oL.r - oL.p * u - oL.q * v;
+        xR = oR.r - oR.p * u - oR.q * v;
     }
 
     // ---------------------------------------------------------------- 4. substitution
-    const T xL = oL.r - oL.p * u - oL.q * v;
-    const T xR = oR.r - oR.p * u - oR.q * v;
     T *de = o.de;
     if constexpr (M > 2) {
         T xn = xR;
