@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/tridiagonal.h"
+#include "tri_adjoint.hpp"
 #include "tri_host.hpp"
 
 namespace tri {
@@ -411,6 +412,48 @@ static int solve_tile_impl(int index, int layout, int64_t n, int64_t batch, T *a
     return err == hipSuccess ? TRIDIAG_OK : hip_fail(err, "tile launch");
 }
 
+
+// ---------------------------------------------------------------- adjoint
+template <typename T>
+static const AdjEntry<T> *choose_adj(int layout, int64_t n, int mode, int algo) {
+    AdjTable<T> tab = adj_table<T>();
+    const bool il = layout == kInterleaved;
+    const AdjEntry<T> *best = nullptr;
+    for (int i = 0; i < tab.count; ++i) {
+        const AdjEntry<T> &e = tab.entries[i];
+        if (e.IL != il || e.mode != mode || e.cap < n) continue;
+        const AlgoSpec sp = algo_spec(algo);
+        const long lanes = 64L * e.NW, w = sizeof(T);
+        const long dyn = sp.iface >= kIfacePCR ? 8 * lanes * w : (sp.xchg_lds ? 6 * lanes * w : 0);
+        if (dyn + 4 * lanes * w > max_lds_per_block()) continue;
+        if (!best || e.cap < best->cap || (e.cap == best->cap && e.M > best->M)) best = &e;
+    }
+    return best;
+}
+
+template <typename T>
+static int adjoint_impl(int mode, int algo, int layout, int64_t n, int64_t batch, const T *a,
+                        const T *b, const T *c, const T *d, const T *x, const T *g, T *xo, T *ga,
+                        T *gb, T *gc, T *gd, hipStream_t st) {
+    if (n < 1 || batch < 0) return fail(TRIDIAG_EINVAL, "bad shape");
+    if (layout != kContiguous && layout != kInterleaved)
+        return fail(TRIDIAG_EINVAL, "adjoint supports layouts 0 and 2");
+    if (!a || !b || !c || !g || !ga || !gb || !gc || !gd || (mode == kFused ? (!d || !xo) : !x))
+        return fail(TRIDIAG_EINVAL, "null pointer");
+    if (batch == 0) return TRIDIAG_OK;
+    const int64_t sstride = layout == kInterleaved ? 1 : n;
+    const int64_t rstride = layout == kInterleaved ? batch : 1;
+    const int32_t bb = buffer_extent<T>(n, batch, sstride, layout);
+    if (bb == 0) return fail(TRIDIAG_EUNSUPPORTED, "adjoint arrays must be below 2 GiB");
+    const AdjEntry<T> *e = choose_adj<T>(layout, n, mode, algo);
+    if (!e) return fail(TRIDIAG_EUNSUPPORTED, "n_eqt=" + std::to_string(n) +
+                                                  " exceeds the on-chip capacity of the adjoint");
+    const AlgoSpec sp = algo_spec(algo);
+    AdjParams<T> p{a, b, c, d, x, g, xo, ga, gb, gc, gd, n, batch, sstride, rstride, bb,
+                   sp.iface, sp.xchg_lds};
+    hipError_t err = e->launch(p, st);
+    return err == hipSuccess ? TRIDIAG_OK : hip_fail(err, "adjoint launch");
+}
 }  // namespace tri
 
 using namespace tri;
@@ -530,18 +573,38 @@ int tridiag_solve_ragged(int precision, int64_t n_sys, const int64_t *offsets_ho
 int tridiag_adjoint(int precision, int layout, int64_t n_eqt, int64_t n_batch, const void *a,
                     const void *b, const void *c, const void *x, const void *g, void *ga,
                     void *gb, void *gc, void *gd, void *stream) {
-    (void)precision; (void)layout; (void)n_eqt; (void)n_batch; (void)a; (void)b; (void)c;
-    (void)x; (void)g; (void)ga; (void)gb; (void)gc; (void)gd; (void)stream;
-    return fail(TRIDIAG_EUNSUPPORTED, "adjoint not built yet");
+    g_last_error.clear();
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == TRIDIAG_SINGLE)
+        return adjoint_impl<float>(kAdjoint, kWM5Reg, layout, n_eqt, n_batch, (const float *)a,
+                                   (const float *)b, (const float *)c, nullptr, (const float *)x,
+                                   (const float *)g, nullptr, (float *)ga, (float *)gb,
+                                   (float *)gc, (float *)gd, st);
+    if (precision == TRIDIAG_DOUBLE)
+        return adjoint_impl<double>(kAdjoint, kWM5Reg, layout, n_eqt, n_batch, (const double *)a,
+                                    (const double *)b, (const double *)c, nullptr,
+                                    (const double *)x, (const double *)g, nullptr, (double *)ga,
+                                    (double *)gb, (double *)gc, (double *)gd, st);
+    return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
 }
 
 int tridiag_solve_fwd_bwd(int precision, int layout, int64_t n_eqt, int64_t n_batch,
                           const void *a, const void *b, const void *c, const void *d,
                           const void *g, void *x, void *ga, void *gb, void *gc, void *gd,
                           void *stream) {
-    (void)precision; (void)layout; (void)n_eqt; (void)n_batch; (void)a; (void)b; (void)c;
-    (void)d; (void)g; (void)x; (void)ga; (void)gb; (void)gc; (void)gd; (void)stream;
-    return fail(TRIDIAG_EUNSUPPORTED, "fused forward+adjoint not built yet");
+    g_last_error.clear();
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == TRIDIAG_SINGLE)
+        return adjoint_impl<float>(kFused, kWM5Reg, layout, n_eqt, n_batch, (const float *)a,
+                                   (const float *)b, (const float *)c, (const float *)d, nullptr,
+                                   (const float *)g, (float *)x, (float *)ga, (float *)gb,
+                                   (float *)gc, (float *)gd, st);
+    if (precision == TRIDIAG_DOUBLE)
+        return adjoint_impl<double>(kFused, kWM5Reg, layout, n_eqt, n_batch, (const double *)a,
+                                    (const double *)b, (const double *)c, (const double *)d,
+                                    nullptr, (const double *)g, (double *)x, (double *)ga,
+                                    (double *)gb, (double *)gc, (double *)gd, st);
+    return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
 }
 
 const char *tridiag_last_error(void) { return g_last_error.c_str(); }

@@ -302,6 +302,135 @@ __device__ __forceinline__ void lane_sweep(SweepOut<T, Sh::NI> &o, const Loader 
     }
 }
 
+// Solves the interface system of a workgroup of tiles from every lane's reduced
+// rows   L: aL u + DLr x_L + ELr x_R = FLr ;  R: ARr x_L + DRr x_R + cR v = FRr
+// and returns x at the lane's first (xL) and last (xR) rows.  Every lane of the
+// workgroup must call it (it synchronises the workgroup).
+template <typename T, int S, int SW, int NWS, int LS, int P, int LANES>
+__device__ __forceinline__ void interface_solve(T aL, T DLr, T ELr, T FLr, T ARr, T DRr, T cR,
+                                                T FRr, int tid, int lane, int sl, int kw, int ws,
+                                                int iface, int xchg_lds, char *tri_dyn_lds,
+                                                T &xL, T &xR) {
+    const int seg = ws * SW + kw;            // segment index inside the system
+    if (iface >= kIfacePCR) {
+        // PCR / PCR + Thomas / CR on the 2P interface rows, through LDS
+        IfaceLds<T> IL_ = iface_lds<T>(tri_dyn_lds, S, 2 * P);
+        Row4<T> rl{aL, DLr, ELr, FLr}, rr{ARr, DRr, cR, FRr};
+        if (iface == kIfacePCR || iface == kIfacePCRThomas) {
+            const int lim = (iface == kIfacePCR) ? P : SW;
+            pcr_interface(IL_, sl, seg, P, lim, rl, rr);
+            if (lim == P) {
+                xL = rl.d * rcp(rl.b);
+                xR = rr.d * rcp(rr.b);
+            } else {
+                IL_.put(sl, 2 * seg, rl);
+                IL_.put(sl, 2 * seg + 1, rr);
+                __syncthreads();
+                strided_thomas(IL_, S, P, SW, tid);
+                xL = IL_.d[sl * 2 * P + 2 * seg];
+                xR = IL_.d[sl * 2 * P + 2 * seg + 1];
+            }
+        } else {
+            IL_.put(sl, 2 * seg, rl);
+            IL_.put(sl, 2 * seg + 1, rr);
+            __syncthreads();
+            cr_solve(IL_, S, tid, LANES);
+            xL = IL_.d[sl * 2 * P + 2 * seg];
+            xR = IL_.d[sl * 2 * P + 2 * seg + 1];
+        }
+        return;
+    }
+    // Wang-Mou: outer form of the segment, merges across lanes, pGE chain across blocks
+    Outer<T> oL, oR;
+    {
+        T inv = rcp(DLr * DRr - ELr * ARr);
+        oL.r = (DRr * FLr - ELr * FRr) * inv;
+        oL.p = (DRr * aL) * inv;
+        oL.q = -(ELr * cR) * inv;
+        oR.r = (DLr * FRr - ARr * FLr) * inv;
+        oR.p = -(ARr * aL) * inv;
+        oR.q = (DLr * cR) * inv;
+    }
+    const int half = iface == kIfaceWMHalf ? 1 : 0;
+    const int BS = SW >> half;                   // segments per merged block
+    const bool lds_x = xchg_lds != 0;            // exchange through LDS instead of bpermute
+    T *xs = reinterpret_cast<T *>(tri_dyn_lds) + (size_t)(tid & ~63) * 6;
+#pragma unroll
+    for (int t = 0; (1 << t) < SW; ++t) {
+        if ((1 << t) >= BS) break;
+        const int h = 1 << t;
+        const int blk = kw >> t;
+        const int pb = (blk & ~1) << t;              // first segment of the merged pair
+        const int srcA = sl % LS + LS * (pb + h - 1); // lane with A's last row
+        const int srcB = sl % LS + LS * (pb + h);     // lane with B's first row
+        T rA, pA, qA, rB, pB, qB;
+        if (lds_x) {
+            xs[lane * 6 + 0] = oL.r; xs[lane * 6 + 1] = oL.p; xs[lane * 6 + 2] = oL.q;
+            xs[lane * 6 + 3] = oR.r; xs[lane * 6 + 4] = oR.p; xs[lane * 6 + 5] = oR.q;
+            wave_lds_sync();
+            rA = xs[srcA * 6 + 3]; pA = xs[srcA * 6 + 4]; qA = xs[srcA * 6 + 5];
+            rB = xs[srcB * 6 + 0]; pB = xs[srcB * 6 + 1]; qB = xs[srcB * 6 + 2];
+            wave_lds_sync();
+        } else {
+            rA = lane_read(oR.r, srcA); pA = lane_read(oR.p, srcA); qA = lane_read(oR.q, srcA);
+            rB = lane_read(oL.r, srcB); pB = lane_read(oL.p, srcB); qB = lane_read(oL.q, srcB);
+        }
+        T RB, PB, QB, RA, PA, QA;
+        wm_pair_coeffs(rA, pA, qA, rB, pB, qB, RB, PB, QB, RA, PA, QA);
+        const bool inA = (blk & 1) == 0;
+        wm_merge_row(oL, inA, RB, PB, QB, RA, PA, QA);
+        wm_merge_row(oR, inA, RB, PB, QB, RA, PA, QA);
+    }
+
+    // pGE chain over the NB = NWS << half merged blocks of each system
+    T u = 0, v = 0;
+    const int NB = NWS << half;
+    if (NB > 1) {
+        __shared__ T sh_blk[S][2 * NWS][6];
+        __shared__ T sh_uv[S][2 * NWS][2];
+        const int bi = ws * (1 << half) + kw / BS;
+        if (kw % BS == 0) {
+            sh_blk[sl][bi][0] = oL.r; sh_blk[sl][bi][1] = oL.p; sh_blk[sl][bi][2] = oL.q;
+        }
+        if (kw % BS == BS - 1) {
+            sh_blk[sl][bi][3] = oR.r; sh_blk[sl][bi][4] = oR.p; sh_blk[sl][bi][5] = oR.q;
+        }
+        __syncthreads();
+        if (tid < S) {
+            const int q = tid;
+            T rho = 0, sig = 0;
+            // forward: x_{E_{w-1}} = rho - sig x_{F_w}
+            for (int w = 0; w < NB; ++w) {
+                T rF = sh_blk[q][w][0], pF = sh_blk[q][w][1], qF = sh_blk[q][w][2];
+                T rE = sh_blk[q][w][3], pE = sh_blk[q][w][4], qE = sh_blk[q][w][5];
+                T inv = rcp(T(1) - pF * sig);
+                T rhoF = (rF - pF * rho) * inv;
+                T sigF = qF * inv;
+                T rhoE = rE - pE * (rho - sig * rhoF);
+                T sigE = qE + pE * sig * sigF;
+                sh_blk[q][w][0] = rhoF; sh_blk[q][w][1] = sigF;
+                sh_blk[q][w][2] = rhoE; sh_blk[q][w][3] = sigE;
+                rho = rhoE; sig = sigE;
+            }
+            // backward: x_{F_w} and x_{E_w} from x_{F_{w+1}}
+            T xFn = 0;
+            sh_uv[q][0][0] = 0;
+            for (int w = NB - 1; w >= 0; --w) {
+                T xE = sh_blk[q][w][2] - sh_blk[q][w][3] * xFn;
+                T xF = sh_blk[q][w][0] - sh_blk[q][w][1] * xFn;
+                sh_uv[q][w][1] = xFn;
+                if (w + 1 < NB) sh_uv[q][w + 1][0] = xE;
+                xFn = xF;
+            }
+        }
+        __syncthreads();
+        u = sh_uv[sl][bi][0];
+        v = sh_uv[sl][bi][1];
+    }
+    xL = oL.r - oL.p * u - oL.q * v;
+    xR = oR.r - oR.p * u - oR.q * v;
+}
+
 // PAD = false: the system fills the tile exactly (n == CAP), loads are not
 // predicated.  PAD = true: n < CAP, rows past the end are identity rows.
 // BIG: 64-bit addressing for arrays of 2 GiB or more (always predicated).
@@ -405,124 +534,9 @@ __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
 
     // ---------------------------------------------------------------- 2-3. interface
     T xL, xR;
-    const int seg = ws * SW + kw;            // segment index inside the system
-    if (prm.iface >= kIfacePCR) {
-        // PCR / PCR + Thomas / CR on the 2P interface rows, through LDS
-        IfaceLds<T> IL_ = iface_lds<T>(tri_dyn_lds, S, 2 * Sh::P);
-        Row4<T> rl{o.aL, DLr, ELr, FLr}, rr{ARr, DRr, o.cR, FRr};
-        if (prm.iface == kIfacePCR || prm.iface == kIfacePCRThomas) {
-            const int lim = (prm.iface == kIfacePCR) ? Sh::P : SW;
-            pcr_interface(IL_, sl, seg, Sh::P, lim, rl, rr);
-            if (lim == Sh::P) {
-                xL = rl.d * rcp(rl.b);
-                xR = rr.d * rcp(rr.b);
-            } else {
-                IL_.put(sl, 2 * seg, rl);
-                IL_.put(sl, 2 * seg + 1, rr);
-                __syncthreads();
-                strided_thomas(IL_, S, Sh::P, SW, tid);
-                xL = IL_.d[sl * 2 * Sh::P + 2 * seg];
-                xR = IL_.d[sl * 2 * Sh::P + 2 * seg + 1];
-            }
-        } else {
-            IL_.put(sl, 2 * seg, rl);
-            IL_.put(sl, 2 * seg + 1, rr);
-            __syncthreads();
-            cr_solve(IL_, S, tid, LANES);
-            xL = IL_.d[sl * 2 * Sh::P + 2 * seg];
-            xR = IL_.d[sl * 2 * Sh::P + 2 * seg + 1];
-        }
-    } else {
-        // Wang-Mou: outer form of the segment, merges across lanes, pGE chain across blocks
-        Outer<T> oL, oR;
-        {
-            T inv = rcp(DLr * DRr - ELr * ARr);
-            oL.r = (DRr * FLr - ELr * FRr) * inv;
-            oL.p = (DRr * o.aL) * inv;
-            oL.q = -(ELr * o.cR) * inv;
-            oR.r = (DLr * FRr - ARr * FLr) * inv;
-            oR.p = -(ARr * o.aL) * inv;
-            oR.q = (DLr * o.cR) * inv;
-        }
-        const int half = prm.iface == kIfaceWMHalf ? 1 : 0;
-        const int BS = SW >> half;                   // segments per merged block
-        const bool lds_x = prm.xchg_lds != 0;        // exchange through LDS instead of bpermute
-        T *xs = reinterpret_cast<T *>(tri_dyn_lds) + (size_t)(tid & ~63) * 6;
-#pragma unroll
-        for (int t = 0; (1 << t) < SW; ++t) {
-            if ((1 << t) >= BS) break;
-            const int h = 1 << t;
-            const int blk = kw >> t;
-            const int pb = (blk & ~1) << t;              // first segment of the merged pair
-            const int srcA = sl % LS + LS * (pb + h - 1); // lane with A's last row
-            const int srcB = sl % LS + LS * (pb + h);     // lane with B's first row
-            T rA, pA, qA, rB, pB, qB;
-            if (lds_x) {
-                xs[lane * 6 + 0] = oL.r; xs[lane * 6 + 1] = oL.p; xs[lane * 6 + 2] = oL.q;
-                xs[lane * 6 + 3] = oR.r; xs[lane * 6 + 4] = oR.p; xs[lane * 6 + 5] = oR.q;
-                wave_lds_sync();
-                rA = xs[srcA * 6 + 3]; pA = xs[srcA * 6 + 4]; qA = xs[srcA * 6 + 5];
-                rB = xs[srcB * 6 + 0]; pB = xs[srcB * 6 + 1]; qB = xs[srcB * 6 + 2];
-                wave_lds_sync();
-            } else {
-                rA = lane_read(oR.r, srcA); pA = lane_read(oR.p, srcA); qA = lane_read(oR.q, srcA);
-                rB = lane_read(oL.r, srcB); pB = lane_read(oL.p, srcB); qB = lane_read(oL.q, srcB);
-            }
-            T RB, PB, QB, RA, PA, QA;
-            wm_pair_coeffs(rA, pA, qA, rB, pB, qB, RB, PB, QB, RA, PA, QA);
-            const bool inA = (blk & 1) == 0;
-            wm_merge_row(oL, inA, RB, PB, QB, RA, PA, QA);
-            wm_merge_row(oR, inA, RB, PB, QB, RA, PA, QA);
-        }
-
-        // pGE chain over the NB = NWS << half merged blocks of each system
-        T u = 0, v = 0;
-        const int NB = NWS << half;
-        if (NB > 1) {
-            __shared__ T sh_blk[S][2 * NWS][6];
-            __shared__ T sh_uv[S][2 * NWS][2];
-            const int bi = ws * (1 << half) + kw / BS;
-            if (kw % BS == 0) {
-                sh_blk[sl][bi][0] = oL.r; sh_blk[sl][bi][1] = oL.p; sh_blk[sl][bi][2] = oL.q;
-            }
-            if (kw % BS == BS - 1) {
-                sh_blk[sl][bi][3] = oR.r; sh_blk[sl][bi][4] = oR.p; sh_blk[sl][bi][5] = oR.q;
-            }
-            __syncthreads();
-            if (tid < S) {
-                const int q = tid;
-                T rho = 0, sig = 0;
-                // forward: x_{E_{w-1}} = rho - sig x_{F_w}
-                for (int w = 0; w < NB; ++w) {
-                    T rF = sh_blk[q][w][0], pF = sh_blk[q][w][1], qF = sh_blk[q][w][2];
-                    T rE = sh_blk[q][w][3], pE = sh_blk[q][w][4], qE = sh_blk[q][w][5];
-                    T inv = rcp(T(1) - pF * sig);
-                    T rhoF = (rF - pF * rho) * inv;
-                    T sigF = qF * inv;
-                    T rhoE = rE - pE * (rho - sig * rhoF);
-                    T sigE = qE + pE * sig * sigF;
-                    sh_blk[q][w][0] = rhoF; sh_blk[q][w][1] = sigF;
-                    sh_blk[q][w][2] = rhoE; sh_blk[q][w][3] = sigE;
-                    rho = rhoE; sig = sigE;
-                }
-                // backward: x_{F_w} and x_{E_w} from x_{F_{w+1}}
-                T xFn = 0;
-                sh_uv[q][0][0] = 0;
-                for (int w = NB - 1; w >= 0; --w) {
-                    T xE = sh_blk[q][w][2] - sh_blk[q][w][3] * xFn;
-                    T xF = sh_blk[q][w][0] - sh_blk[q][w][1] * xFn;
-                    sh_uv[q][w][1] = xFn;
-                    if (w + 1 < NB) sh_uv[q][w + 1][0] = xE;
-                    xFn = xF;
-                }
-            }
-            __syncthreads();
-            u = sh_uv[sl][bi][0];
-            v = sh_uv[sl][bi][1];
-        }
-        xL = oL.r - oL.p * u - oL.q * v;
-        xR = oR.r - oR.p * u - oR.q * v;
-    }
+    interface_solve<T, S, SW, NWS, LS, Sh::P, LANES>(o.aL, DLr, ELr, FLr, ARr, DRr, o.cR, FRr, tid,
+                                                     lane, sl, kw, ws, prm.iface, prm.xchg_lds,
+                                                     tri_dyn_lds, xL, xR);
 
     // ---------------------------------------------------------------- 4. substitution
     T *de = o.de;
