@@ -133,11 +133,37 @@ int tridiag_algo_layout(int precision, int algo);
 int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch,
                   void *a, void *b, void *c, void *d, void *stream);
 
-/* Ragged batch (layout 0 per system, systems concatenated): system s holds rows
- * [offsets[s], offsets[s+1]) of a, b, c, d.  offsets is a device array of
- * n_sys+1 int64.  One persistent launch; x -> d.  plan_bytes of device memory
- * at `work` (from tridiag_ragged_work_size) hold the schedule. */
+/* Ragged batch: systems of uneven size, contiguous per system and concatenated.
+ * System s holds rows [offsets[s], offsets[s+1]) of a, b, c, d (a_0 and c_{n-1}
+ * of each system are ignored); x replaces d.  Sizes 0..8192 rows.  The systems
+ * are solved by one persistent launch whose work list (size buckets, a tile
+ * shape and an interface solver per bucket, largest bucket first) is planned on
+ * the host and kept in `work`, a device buffer of tridiag_ragged_work_size(n_sys)
+ * bytes.
+ *
+ *   tridiag_ragged_plan   builds the work list from the host copy of the
+ *                         offsets and uploads it (synchronises `stream`; do not
+ *                         call it during graph capture).  shape = -1 and
+ *                         iface = -1 take each bucket's choice from the
+ *                         autotuned table (tuning_gfx950.txt, lines
+ *                         "R precision n_max shape iface"); a shape or iface
+ *                         >= 0 forces it for every system.
+ *   tridiag_ragged_solve  launches the solve for a planned work buffer (two
+ *                         kernels, no host synchronisation, no allocation:
+ *                         capturable into a graph and replayable).  Concurrent
+ *                         solves on one work buffer race.
+ *   tridiag_solve_ragged  plan + solve.
+ *
+ * No counterpart in the reference, whose entry points take one n_eqt for the
+ * whole batch (Tridiagonal_CUDA/tridiagonal.h:8-57). */
 int64_t tridiag_ragged_work_size(int64_t n_sys);
+int tridiag_ragged_shape_count(void);
+int tridiag_ragged_shape_info(int index, int *M, int *NWS, int *S, int64_t *cap);
+int tridiag_ragged_bucket(int precision, int64_t n, int *shape, int *iface);
+int tridiag_ragged_plan(int precision, int64_t n_sys, const int64_t *offsets_host, int shape,
+                        int iface, void *work, void *stream);
+int tridiag_ragged_solve(int precision, int64_t n_sys, const int64_t *offsets_dev, void *a,
+                         void *b, void *c, void *d, void *work, void *stream);
 int tridiag_solve_ragged(int precision, int64_t n_sys, const int64_t *offsets_host,
                          const int64_t *offsets_dev, void *a, void *b, void *c, void *d,
                          void *work, void *stream);

@@ -166,7 +166,7 @@ OR_EXPORT int oracle_thomas_ragged_f64(const double *a, const double *b, const d
     int64_t nmax = 0;
     for (int64_t s = 0; s < nsys; ++s) {
         int64_t len = offsets[s + 1] - offsets[s];
-        if (len <= 0) return 1;
+        if (len < 0) return 1;   /* empty systems are skipped */
         if (len > nmax) nmax = len;
     }
     int err = 0;
@@ -180,7 +180,8 @@ OR_EXPORT int oracle_thomas_ragged_f64(const double *a, const double *b, const d
 #pragma omp for schedule(dynamic, 64)
             for (int64_t s = 0; s < nsys; ++s) {
                 int64_t o = offsets[s];
-                thomas_one_f64(a + o, b + o, c + o, d + o, x + o, cw, offsets[s + 1] - o, 1);
+                if (offsets[s + 1] > o)
+                    thomas_one_f64(a + o, b + o, c + o, d + o, x + o, cw, offsets[s + 1] - o, 1);
             }
             free(cw);
         }
@@ -195,7 +196,7 @@ OR_EXPORT int oracle_thomas_ragged_f32(const float *a, const float *b, const flo
     int64_t nmax = 0;
     for (int64_t s = 0; s < nsys; ++s) {
         int64_t len = offsets[s + 1] - offsets[s];
-        if (len <= 0) return 1;
+        if (len < 0) return 1;   /* empty systems are skipped */
         if (len > nmax) nmax = len;
     }
     int err = 0;
@@ -209,7 +210,8 @@ OR_EXPORT int oracle_thomas_ragged_f32(const float *a, const float *b, const flo
 #pragma omp for schedule(dynamic, 64)
             for (int64_t s = 0; s < nsys; ++s) {
                 int64_t o = offsets[s];
-                thomas_one_f32(a + o, b + o, c + o, d + o, x + o, cw, offsets[s + 1] - o, 1);
+                if (offsets[s + 1] > o)
+                    thomas_one_f32(a + o, b + o, c + o, d + o, x + o, cw, offsets[s + 1] - o, 1);
             }
             free(cw);
         }

@@ -278,6 +278,19 @@ def test_ragged_matches_per_system(oracle_mod):
     assert np.max(np.abs(x32 - x)) < 1e-5
 
 
+def test_ragged_skips_empty_systems(oracle_mod):
+    sizes = np.array([3, 0, 0, 5, 0, 1], dtype=np.int64)
+    off = oracle_mod.offsets_from_sizes(sizes)
+    a, b, c, d = oracle_mod.gen_dd_ragged(off, seed=9)
+    x = oracle_mod.thomas_ragged(a, b, c, d, off)
+    for s in np.nonzero(sizes)[0]:
+        sl = slice(off[s], off[s + 1])
+        A = dense(a[sl], b[sl], c[sl], int(sizes[s]))
+        assert np.allclose(x[sl], np.linalg.solve(A, d[sl]), rtol=1e-13, atol=1e-14)
+    with pytest.raises(RuntimeError):
+        oracle_mod.thomas_ragged(a, b, c, d, np.array([0, 4, 2, 9], dtype=np.int64))
+
+
 def test_ragged_sizes_distribution(oracle_mod):
     sizes = oracle_mod.ragged_sizes(65536, 32, 8192, seed=2024)
     assert sizes.min() >= 32 and sizes.max() <= 8192

@@ -30,7 +30,9 @@ EXTENDED_SYMBOLS = [
     "tridiag_algo_count", "tridiag_algo_name", "tridiag_algo_layout", "tridiag_solve",
     "tridiag_ragged_work_size", "tridiag_solve_ragged", "tridiag_adjoint",
     "tridiag_solve_fwd_bwd", "tridiag_last_error", "tridiag_plan", "tridiag_tile_count",
-    "tridiag_tile_info", "tridiag_solve_tile",
+    "tridiag_tile_info", "tridiag_solve_tile", "tridiag_ragged_shape_count",
+    "tridiag_ragged_shape_info", "tridiag_ragged_bucket", "tridiag_ragged_plan",
+    "tridiag_ragged_solve",
 ]
 
 
@@ -81,6 +83,16 @@ def lib():
     L.tridiag_tile_info.restype = ci
     L.tridiag_solve_tile.argtypes = [ci, ci, ci, i64, i64, vp, vp, vp, vp, vp]
     L.tridiag_solve_tile.restype = ci
+    L.tridiag_ragged_shape_count.argtypes = []
+    L.tridiag_ragged_shape_count.restype = ci
+    L.tridiag_ragged_shape_info.argtypes = [ci, pi, pi, pi, ctypes.POINTER(i64)]
+    L.tridiag_ragged_shape_info.restype = ci
+    L.tridiag_ragged_bucket.argtypes = [ci, i64, pi, pi]
+    L.tridiag_ragged_bucket.restype = ci
+    L.tridiag_ragged_plan.argtypes = [ci, i64, vp, ci, ci, vp, vp]
+    L.tridiag_ragged_plan.restype = ci
+    L.tridiag_ragged_solve.argtypes = [ci, i64, vp, vp, vp, vp, vp, vp, vp]
+    L.tridiag_ragged_solve.restype = ci
     _lib = L
     return L
 
@@ -121,3 +133,24 @@ def tiles(precision):
                         padded=bool(pad.value), nontemporal=bool(nt.value),
                         transposed=bool(xp.value)))
     return out
+
+
+def ragged_shapes():
+    """Shapes of the ragged persistent kernel: rows per lane M, waves per system NWS,
+    systems per workgroup S and capacity 64*NWS*M."""
+    L = lib()
+    out = []
+    for i in range(L.tridiag_ragged_shape_count()):
+        M, NWS, S = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        cap = ctypes.c_int64()
+        check(L.tridiag_ragged_shape_info(i, ctypes.byref(M), ctypes.byref(NWS), ctypes.byref(S),
+                                          ctypes.byref(cap)))
+        out.append(dict(index=i, M=M.value, NWS=NWS.value, S=S.value, cap=cap.value))
+    return out
+
+
+def ragged_bucket(precision, n):
+    """(shape index, interface solver id) the planner picks for a system of n rows."""
+    sh, fi = ctypes.c_int(), ctypes.c_int()
+    check(lib().tridiag_ragged_bucket(precision, n, ctypes.byref(sh), ctypes.byref(fi)))
+    return sh.value, fi.value

@@ -17,6 +17,7 @@
 #include "../../include/tridiagonal.h"
 #include "tri_adjoint.hpp"
 #include "tri_host.hpp"
+#include "tri_ragged.hpp"
 
 namespace tri {
 
@@ -61,6 +62,14 @@ struct TuneRow {
     int M, NW, S, vec, alds, nt, xp;
 };
 static std::vector<TuneRow> g_tune;
+// Ragged buckets: lines "R precision n_max shape iface" (shape index of the
+// ragged kernel's shape table, interface solver id).
+struct RaggedTuneRow {
+    int prec;
+    long long nmax;
+    int shape, iface;
+};
+static std::vector<RaggedTuneRow> g_rtune;
 static std::once_flag g_tune_once;
 
 static void load_tuning() {
@@ -82,6 +91,12 @@ static void load_tuning() {
     char line[256];
     while (fgets(line, sizeof line, f)) {
         if (line[0] == '#') continue;
+        if (line[0] == 'R') {
+            RaggedTuneRow q;
+            if (sscanf(line + 1, "%d %lld %d %d", &q.prec, &q.nmax, &q.shape, &q.iface) == 4)
+                g_rtune.push_back(q);
+            continue;
+        }
         TuneRow r;
         if (sscanf(line, "%d %d %d %lld %d %d %d %d %d %d %d", &r.prec, &r.layout, &r.family,
                    &r.nmax, &r.M, &r.NW, &r.S, &r.vec, &r.alds, &r.nt, &r.xp) == 11)
@@ -90,6 +105,8 @@ static void load_tuning() {
     fclose(f);
     std::stable_sort(g_tune.begin(), g_tune.end(),
                      [](const TuneRow &x, const TuneRow &y) { return x.nmax < y.nmax; });
+    std::stable_sort(g_rtune.begin(), g_rtune.end(),
+                     [](const RaggedTuneRow &x, const RaggedTuneRow &y) { return x.nmax < y.nmax; });
 }
 
 int max_lds_per_block() {
@@ -413,6 +430,140 @@ static int solve_tile_impl(int index, int layout, int64_t n, int64_t batch, T *a
 }
 
 
+// ---------------------------------------------------------------- ragged batches
+// Default shape of a system of n rows: the smallest capacity that holds it; among
+// equal capacities the one with the fewest waves per system (no cross-wave chain).
+static int ragged_default_shape(int64_t n) {
+    int best = -1;
+    for (int k = 0; k < kRaggedShapes; ++k) {
+        if (ragged_cap(k) < n) continue;
+        if (best < 0 || ragged_cap(k) < ragged_cap(best) ||
+            (ragged_cap(k) == ragged_cap(best) && kRaggedShapeTab[k].NWS < kRaggedShapeTab[best].NWS))
+            best = k;
+    }
+    return best;
+}
+
+// Bucket choice for a system of n rows: the autotuned row of its size bucket,
+// else the default shape with the Wang-Mou interface solver.
+static void ragged_choice(int prec, int64_t n, int &shape, int &iface) {
+    std::call_once(g_tune_once, load_tuning);
+    for (const RaggedTuneRow &r : g_rtune) {
+        if (r.prec != prec || n > r.nmax) continue;
+        if (r.shape < 0 || r.shape >= kRaggedShapes || ragged_cap(r.shape) < n) continue;
+        if (r.iface < kIfaceWM || r.iface > kIfaceCR) continue;
+        shape = r.shape;
+        iface = r.iface;
+        return;
+    }
+    shape = ragged_default_shape(n);
+    iface = kIfaceWM;
+}
+
+static int64_t ragged_work_bytes(int64_t n_sys) {
+    const int64_t raw = kRaggedHeaderBytes + (int64_t)(4 * kRaggedItemInts + 4) * n_sys;
+    return (raw + 255) & ~(int64_t)255;
+}
+
+// Host planner: validates the offsets, buckets the systems, orders the work
+// items largest bucket first and copies the schedule into `work`.
+static int ragged_plan_impl(int prec, int64_t n_sys, const int64_t *off, int force_shape,
+                            int force_iface, void *work, hipStream_t st) {
+    if (prec != TRIDIAG_SINGLE && prec != TRIDIAG_DOUBLE)
+        return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
+    if (n_sys < 0 || n_sys > 0x7fffffff) return fail(TRIDIAG_EINVAL, "n_sys out of range");
+    if (!work) return fail(TRIDIAG_EINVAL, "null work buffer");
+    if (n_sys > 0 && !off) return fail(TRIDIAG_EINVAL, "null offsets");
+    if (force_shape >= kRaggedShapes || force_shape < -1)
+        return fail(TRIDIAG_EINVAL, "shape index out of range");
+    if (force_iface > kIfaceCR || force_iface < -1)
+        return fail(TRIDIAG_EINVAL, "interface solver id out of range");
+    if (n_sys > 0 && off[0] < 0) return fail(TRIDIAG_EINVAL, "offsets[0] is negative");
+    // (shape, iface) bucket of every system
+    std::vector<int32_t> key((size_t)n_sys, -1);
+    std::vector<int64_t> cnt((size_t)kRaggedShapes * 5, 0);
+    for (int64_t s = 0; s < n_sys; ++s) {
+        const int64_t n = off[s + 1] - off[s];
+        if (n < 0) return fail(TRIDIAG_EINVAL, "offsets must be non-decreasing");
+        if (n == 0) continue;
+        if (n > kRaggedMaxN)
+            return fail(TRIDIAG_EUNSUPPORTED, "system " + std::to_string(s) + " has " +
+                                                  std::to_string(n) + " rows; the ragged solve "
+                                                  "holds up to " + std::to_string(kRaggedMaxN));
+        int shape, iface;
+        ragged_choice(prec, n, shape, iface);
+        if (force_shape >= 0) {
+            if (ragged_cap(force_shape) < n)
+                return fail(TRIDIAG_EINVAL, "system " + std::to_string(s) +
+                                                " exceeds the capacity of the forced shape");
+            shape = force_shape;
+        }
+        if (force_iface >= 0) iface = force_iface;
+        key[(size_t)s] = shape * 5 + iface;
+        ++cnt[(size_t)key[(size_t)s]];
+    }
+    // buckets in order of decreasing capacity (then decreasing systems per item)
+    std::vector<int> order(cnt.size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+    std::stable_sort(order.begin(), order.end(), [](int x, int y) {
+        const int sx = x / 5, sy = y / 5;
+        if (ragged_cap(sx) != ragged_cap(sy)) return ragged_cap(sx) > ragged_cap(sy);
+        return kRaggedShapeTab[sx].S < kRaggedShapeTab[sy].S;
+    });
+    std::vector<int64_t> first(cnt.size(), 0);
+    int64_t pos = 0;
+    for (int k : order) {
+        first[(size_t)k] = pos;
+        pos += cnt[(size_t)k];
+    }
+    const int64_t n_active = pos;
+    const int64_t bytes = ragged_work_bytes(n_sys);
+    std::vector<char> host((size_t)bytes, 0);
+    int32_t *items = reinterpret_cast<int32_t *>(host.data() + kRaggedHeaderBytes);
+    int32_t *perm = items + (size_t)kRaggedItemInts * n_sys;
+    {
+        std::vector<int64_t> fill(first);
+        for (int64_t s = 0; s < n_sys; ++s)
+            if (key[(size_t)s] >= 0) perm[fill[(size_t)key[(size_t)s]]++] = (int32_t)s;
+    }
+    int64_t nitems = 0;
+    for (int k : order) {
+        const int S = kRaggedShapeTab[k / 5].S;
+        for (int64_t i = 0; i < cnt[(size_t)k]; i += S) {
+            int32_t *w = items + (size_t)kRaggedItemInts * nitems++;
+            w[0] = k / 5;
+            w[1] = k % 5;
+            w[2] = (int32_t)(first[(size_t)k] + i);
+            w[3] = (int32_t)std::min<int64_t>(S, cnt[(size_t)k] - i);
+        }
+    }
+    (void)n_active;
+    RaggedHeader h{kRaggedMagic, n_sys, nitems, 0, prec};
+    memcpy(host.data(), &h, sizeof h);
+    hipError_t e = hipMemcpyAsync(work, host.data(), (size_t)bytes, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "ragged plan upload");
+    e = hipStreamSynchronize(st);
+    return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "ragged plan upload");
+}
+
+template <typename T>
+static int ragged_solve_impl(int prec, int64_t n_sys, const int64_t *off_dev, T *a, T *b, T *c,
+                             T *d, void *work, hipStream_t st) {
+    if (n_sys < 0 || n_sys > 0x7fffffff) return fail(TRIDIAG_EINVAL, "n_sys out of range");
+    if (n_sys == 0) return TRIDIAG_OK;
+    if (!work || !off_dev || !a || !b || !c || !d) return fail(TRIDIAG_EINVAL, "null pointer");
+    char *w = reinterpret_cast<char *>(work);
+    RaggedParams<T> p{a, b, c, d, off_dev, reinterpret_cast<RaggedHeader *>(w),
+                      reinterpret_cast<const int32_t *>(w + kRaggedHeaderBytes),
+                      reinterpret_cast<const int32_t *>(w + kRaggedHeaderBytes) +
+                          (size_t)kRaggedItemInts * n_sys,
+                      n_sys, prec};
+    hipError_t e;
+    if constexpr (sizeof(T) == 4) e = launch_ragged_f32(p, ragged_grid_f32(), st);
+    else e = launch_ragged_f64(p, ragged_grid_f64(), st);
+    return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "ragged launch");
+}
+
 // ---------------------------------------------------------------- adjoint
 template <typename T>
 static const AdjEntry<T> *choose_adj(int layout, int64_t n, int mode, int algo) {
@@ -560,14 +711,58 @@ int tridiag_solve_tile(int precision, int index, int layout, int64_t n_eqt, int6
     return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
 }
 
-int64_t tridiag_ragged_work_size(int64_t n_sys) { return n_sys < 0 ? 0 : 64 * (n_sys + 1); }
+int64_t tridiag_ragged_work_size(int64_t n_sys) { return n_sys < 0 ? 0 : ragged_work_bytes(n_sys); }
+
+int tridiag_ragged_shape_count(void) { return kRaggedShapes; }
+
+int tridiag_ragged_shape_info(int index, int *M, int *NWS, int *S, int64_t *cap) {
+    g_last_error.clear();
+    if (index < 0 || index >= kRaggedShapes) return fail(TRIDIAG_EINVAL, "shape index out of range");
+    if (M) *M = kRaggedShapeTab[index].M;
+    if (NWS) *NWS = kRaggedShapeTab[index].NWS;
+    if (S) *S = kRaggedShapeTab[index].S;
+    if (cap) *cap = ragged_cap(index);
+    return TRIDIAG_OK;
+}
+
+int tridiag_ragged_bucket(int precision, int64_t n, int *shape, int *iface) {
+    g_last_error.clear();
+    if (precision != TRIDIAG_SINGLE && precision != TRIDIAG_DOUBLE)
+        return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
+    if (n < 1 || n > kRaggedMaxN) return fail(TRIDIAG_EUNSUPPORTED, "n outside the ragged range");
+    int s, f;
+    ragged_choice(precision, n, s, f);
+    if (shape) *shape = s;
+    if (iface) *iface = f;
+    return TRIDIAG_OK;
+}
+
+int tridiag_ragged_plan(int precision, int64_t n_sys, const int64_t *offsets_host, int shape,
+                        int iface, void *work, void *stream) {
+    g_last_error.clear();
+    return ragged_plan_impl(precision, n_sys, offsets_host, shape, iface, work,
+                            (hipStream_t)stream);
+}
+
+int tridiag_ragged_solve(int precision, int64_t n_sys, const int64_t *offsets_dev, void *a,
+                         void *b, void *c, void *d, void *work, void *stream) {
+    g_last_error.clear();
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == TRIDIAG_SINGLE)
+        return ragged_solve_impl<float>(precision, n_sys, offsets_dev, (float *)a, (float *)b,
+                                        (float *)c, (float *)d, work, st);
+    if (precision == TRIDIAG_DOUBLE)
+        return ragged_solve_impl<double>(precision, n_sys, offsets_dev, (double *)a, (double *)b,
+                                         (double *)c, (double *)d, work, st);
+    return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
+}
 
 int tridiag_solve_ragged(int precision, int64_t n_sys, const int64_t *offsets_host,
                          const int64_t *offsets_dev, void *a, void *b, void *c, void *d,
                          void *work, void *stream) {
-    (void)precision; (void)n_sys; (void)offsets_host; (void)offsets_dev; (void)a; (void)b;
-    (void)c; (void)d; (void)work; (void)stream;
-    return fail(TRIDIAG_EUNSUPPORTED, "ragged solve not built yet");
+    const int rc = tridiag_ragged_plan(precision, n_sys, offsets_host, -1, -1, work, stream);
+    if (rc != TRIDIAG_OK) return rc;
+    return tridiag_ragged_solve(precision, n_sys, offsets_dev, a, b, c, d, work, stream);
 }
 
 int tridiag_adjoint(int precision, int layout, int64_t n_eqt, int64_t n_batch, const void *a,

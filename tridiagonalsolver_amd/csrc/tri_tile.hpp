@@ -156,7 +156,7 @@ template <typename T, int CH, int V, bool NT> struct FastLoader {
 
 // Padded tiles (n < CAP): the same buffer addressing, rows past the end of the
 // system are identity rows (0, 1, 0, 0) and are not read.
-template <typename T, int CH, int V> struct PredLoader {
+template <typename T, int CH, int V, bool NT = false> struct PredLoader {
     __amdgpu_buffer_rsrc_t ra, rb, rc, rd;
     int vo, rowb;
     int64_t row0, n;
@@ -166,10 +166,10 @@ template <typename T, int CH, int V> struct PredLoader {
             const int64_t r = row0 + j0 + v0;
             const int so = (j0 + v0) * rowb;
             if (V > 1 && r + V <= n) {
-                u32x4 xa = __builtin_amdgcn_raw_buffer_load_b128(ra, vo, so, 0);
-                u32x4 xb = __builtin_amdgcn_raw_buffer_load_b128(rb, vo, so, 0);
-                u32x4 xc = __builtin_amdgcn_raw_buffer_load_b128(rc, vo, so, 0);
-                u32x4 xd = __builtin_amdgcn_raw_buffer_load_b128(rd, vo, so, 0);
+                u32x4 xa = __builtin_amdgcn_raw_buffer_load_b128(ra, vo, so, cpol<NT>());
+                u32x4 xb = __builtin_amdgcn_raw_buffer_load_b128(rb, vo, so, cpol<NT>());
+                u32x4 xc = __builtin_amdgcn_raw_buffer_load_b128(rc, vo, so, cpol<NT>());
+                u32x4 xd = __builtin_amdgcn_raw_buffer_load_b128(rd, vo, so, cpol<NT>());
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
                     q[0][v0 + v] = reinterpret_cast<const T *>(&xa)[v];
@@ -182,10 +182,10 @@ template <typename T, int CH, int V> struct PredLoader {
                 for (int v = 0; v < V; ++v) {
                     if (r + v < n) {
                         const int sov = so + v * rowb;
-                        q[0][v0 + v] = bload<T>(ra, vo, sov);
-                        q[1][v0 + v] = bload<T>(rb, vo, sov);
-                        q[2][v0 + v] = bload<T>(rc, vo, sov);
-                        q[3][v0 + v] = bload<T>(rd, vo, sov);
+                        q[0][v0 + v] = bload<T, NT>(ra, vo, sov);
+                        q[1][v0 + v] = bload<T, NT>(rb, vo, sov);
+                        q[2][v0 + v] = bload<T, NT>(rc, vo, sov);
+                        q[3][v0 + v] = bload<T, NT>(rd, vo, sov);
                     } else {
                         q[0][v0 + v] = 0; q[1][v0 + v] = 1; q[2][v0 + v] = 0; q[3][v0 + v] = 0;
                     }

@@ -61,19 +61,55 @@ def solve(a, b, c, d, algo=ALGO_AUTO, layout=CONTIGUOUS, n=None, batch=None):
     return a if layout == PACKED else d
 
 
-def solve_ragged(a, b, c, d, offsets, work=None):
-    """Ragged batch: system s = rows [offsets[s], offsets[s+1]) of the 1-D tensors.
-    `offsets` is an int64 CPU tensor or list; x is written to d."""
+class RaggedPlan:
+    """Schedule of a ragged batch: system s = rows [offsets[s], offsets[s+1]) of 1-D
+    tensors a, b, c, d (contiguous per system, concatenated), planned once on the
+    host (size buckets, per-bucket shape and interface solver) and held in a device
+    work buffer.  `solve` only launches kernels: it allocates nothing and can be
+    captured into a graph and replayed.  Solves on one plan must not overlap.
+
+    shape / iface: -1 = per-bucket choice from the tuning table; >= 0 forces one
+    shape of the ragged kernel (ragged_shapes()) / interface solver for all systems.
+    """
+
+    def __init__(self, offsets, dtype=torch.float32, device=None, shape=-1, iface=-1):
+        self.precision = _precision(torch.empty(0, dtype=dtype))
+        self.offsets_host = torch.as_tensor(offsets, dtype=torch.int64).contiguous().cpu()
+        self.n_sys = self.offsets_host.numel() - 1
+        if self.n_sys < 0:
+            raise ValueError("offsets needs at least one entry")
+        device = torch.device(device if device is not None else "cuda")
+        self.device = device
+        self.offsets = self.offsets_host.to(device)
+        self.work = torch.empty(int(lib().tridiag_ragged_work_size(self.n_sys)),
+                                dtype=torch.uint8, device=device)
+        check(lib().tridiag_ragged_plan(self.precision, self.n_sys,
+                                        ctypes.c_void_p(self.offsets_host.data_ptr()), shape,
+                                        iface, _ptr(self.work), _stream_handle(device)))
+
+    @property
+    def rows(self):
+        return int(self.offsets_host[-1]) if self.n_sys >= 0 else 0
+
+    def solve(self, a, b, c, d):
+        """Solves every system in place (x -> d) on the current stream."""
+        _check_tensors([a, b, c, d])
+        if _precision(a) != self.precision:
+            raise ValueError("tensor dtype does not match the plan's precision")
+        for t in (a, b, c, d):
+            if t.numel() < self.rows:
+                raise ValueError("tensors hold fewer rows than offsets[-1]")
+        check(lib().tridiag_ragged_solve(self.precision, self.n_sys, _ptr(self.offsets), _ptr(a),
+                                         _ptr(b), _ptr(c), _ptr(d), _ptr(self.work),
+                                         _stream_handle(a.device)))
+        return d
+
+
+def solve_ragged(a, b, c, d, offsets):
+    """Ragged batch in one call (plan + solve): system s = rows
+    [offsets[s], offsets[s+1]) of the 1-D tensors; x is written to d."""
     _check_tensors([a, b, c, d])
-    off_host = torch.as_tensor(offsets, dtype=torch.int64).contiguous().cpu()
-    nsys = off_host.numel() - 1
-    off_dev = off_host.to(a.device, non_blocking=False)
-    if work is None:
-        work = torch.empty(int(lib().tridiag_ragged_work_size(nsys)), dtype=torch.uint8,
-                           device=a.device)
-    check(lib().tridiag_solve_ragged(_precision(a), nsys, ctypes.c_void_p(off_host.data_ptr()),
-                                     _ptr(off_dev), _ptr(a), _ptr(b), _ptr(c), _ptr(d),
-                                     _ptr(work), _stream_handle(a.device)))
+    RaggedPlan(offsets, dtype=a.dtype, device=a.device).solve(a, b, c, d)
     return d
 
 
