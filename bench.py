@@ -158,6 +158,199 @@ def run_workload(torch, T, name, dtname, layout, steps, warmup):
     return r
 
 
+def _event_time(torch, fn, steps, stream):
+    """Runs fn() `steps` times between two HIP events on `stream`; returns
+    (wall ms per call, event ms per call)."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, e0.elapsed_time(e1) / steps
+
+
+def _ragged_relerr(x, xref, off):
+    import numpy as np
+    starts = off[:-1][np.diff(off) > 0]
+    err = np.maximum.reduceat(np.abs(x - xref), starts)
+    ref = np.maximum.reduceat(np.abs(xref), starts)
+    return float(np.max(err / np.where(ref > 0, ref, 1.0)))
+
+
+def run_ragged(torch, T, steps, warmup, n_sys=65536, nmin=32, nmax=8192):
+    """Ragged batch, FP32: n_sys systems with n log-uniform in [nmin, nmax] (seeded),
+    concatenated in order of size so that the comparison path can issue one plain
+    launch per distinct n over the same buffers.  Reports the persistent single
+    launch (eager and replayed from a captured graph) against the per-size plain
+    launches (eager and captured)."""
+    import ctypes
+    import numpy as np
+    import oracle
+    log("prepare ragged")
+    sizes = np.sort(oracle.ragged_sizes(n_sys, nmin, nmax, seed=2024))
+    off = oracle.offsets_from_sizes(sizes)
+    U = int(off[-1])
+    arrs = [v.astype(np.float32) for v in oracle.gen_dd_ragged(off, seed=2025)]
+    dev = torch.device("cuda:0")
+    src = [torch.from_numpy(v).to(dev) for v in arrs]
+    work = [t.clone() for t in src]
+    stream = torch.cuda.current_stream()
+    plan = T.RaggedPlan(off, dtype=torch.float32, device=dev)
+    alg_bytes = 5 * 4 * U + 8 * (n_sys + 1)
+
+    # parity of both paths on fresh inputs
+    log("ragged parity")
+    xref = oracle.thomas_ragged(*arrs, off)
+    plan.solve(*work)
+    torch.cuda.synchronize()
+    err_ragged = _ragged_relerr(work[3].cpu().numpy().astype(np.float64), xref, off)
+    L = T.lib()
+    h = ctypes.c_void_p(stream.cuda_stream)
+    uniq, first, counts = np.unique(sizes, return_index=True, return_counts=True)
+    ptrs = [t.data_ptr() for t in work]
+
+    def plain_args(handle):
+        out = []
+        for n, f, k in zip(uniq.tolist(), first.tolist(), counts.tolist()):
+            o = int(off[f]) * 4
+            out.append((0, -1, 0, n, k) + tuple(ctypes.c_void_p(p + o) for p in ptrs) +
+                       (handle,))
+        return out
+    pargs = plain_args(h)
+
+    def plain_sweep(args=pargs):
+        rc = 0
+        for a in args:
+            rc |= L.tridiag_solve(*a)
+        return rc
+    work[3].copy_(src[3])
+    T.check(plain_sweep())
+    torch.cuda.synchronize()
+    err_plain = _ragged_relerr(work[3].cpu().numpy().astype(np.float64), xref, off)
+    del xref
+
+    log("ragged timing")
+    for _ in range(warmup):
+        plan.solve(*work)
+    wall_r, ev_r = _event_time(torch, lambda: plan.solve(*work), steps, stream)
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(stream)
+    with torch.cuda.graph(g, stream=s):
+        plan.solve(*work)
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        g.replay()
+    wall_rg, ev_rg = _event_time(torch, g.replay, steps, stream)
+
+    log("plain per-size timing (%d launches per sweep)" % len(pargs))
+    psteps = max(2, min(steps, 10))
+    for _ in range(min(warmup, 2)):
+        plain_sweep()
+    wall_p, ev_p = _event_time(torch, plain_sweep, psteps, stream)
+    gp = torch.cuda.CUDAGraph()
+    pargs_s = plain_args(ctypes.c_void_p(s.cuda_stream))
+    with torch.cuda.graph(gp, stream=s):
+        T.check(plain_sweep(pargs_s))
+    torch.cuda.synchronize()
+    for _ in range(min(warmup, 2)):
+        gp.replay()
+    wall_pg, ev_pg = _event_time(torch, gp.replay, psteps, stream)
+    del g, gp, work, src
+    torch.cuda.empty_cache()
+
+    achieved = alg_bytes / (ev_r * 1e-3) / 1e9
+    r = {
+        "systems": n_sys, "unknowns": U, "distinct_n": int(len(uniq)), "dtype": "f32",
+        "sizes": "n = round(exp(U[ln %d, ln %d])), seed 2024, ordered by size" % (nmin, nmax),
+        "persistent": {"ms_per_step": wall_r, "kernel_ms": ev_r,
+                       "unknowns_per_s": U / (wall_r * 1e-3), "relerr": err_ragged},
+        "persistent_graph": {"ms_per_step": wall_rg, "kernel_ms": ev_rg,
+                             "unknowns_per_s": U / (wall_rg * 1e-3)},
+        "plain_per_size": {"launches": int(len(uniq)), "ms_per_step": wall_p, "gpu_ms": ev_p,
+                           "unknowns_per_s": U / (wall_p * 1e-3), "relerr": err_plain},
+        "plain_per_size_graph": {"ms_per_step": wall_pg, "gpu_ms": ev_pg,
+                                 "unknowns_per_s": U / (wall_pg * 1e-3)},
+        "speedup_vs_plain_graph": wall_pg / wall_rg,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic("ragged_f32")},
+    }
+    log("ragged", json.dumps(r))
+    return r
+
+
+def run_fwd_bwd(torch, T, steps, warmup, n=512, batch=8192, layout=0):
+    """Differentiable solve, FP32, n x batch: plain forward solve vs the fused
+    forward + adjoint kernel (and the stand-alone adjoint), inputs resident."""
+    import ctypes
+    import numpy as np
+    import oracle
+    log("prepare fwd/bwd")
+    dev = torch.device("cuda:0")
+    a, b, c, d = oracle.gen_dd(n, batch, layout=layout, seed=41)
+    w = np.random.default_rng(42).standard_normal(n * batch)          # L = sum(w * x)
+    arrs = [v.astype(np.float32) for v in (a, b, c, d)]
+    ts = [torch.from_numpy(v).to(dev) for v in arrs]
+    g = torch.from_numpy(w.astype(np.float32)).to(dev)
+    outs = [torch.empty_like(ts[0]) for _ in range(5)]
+    xf = ts[3].clone()
+    stream = torch.cuda.current_stream()
+    h = ctypes.c_void_p(stream.cuda_stream)
+    L = T.lib()
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    fwd_args = (0, -1, layout, n, batch, P(ts[0]), P(ts[1]), P(ts[2]), P(xf), h)
+    fb_args = (0, layout, n, batch, P(ts[0]), P(ts[1]), P(ts[2]), P(ts[3]), P(g)) + \
+        tuple(P(o) for o in outs) + (h,)
+    adj_outs = [torch.empty_like(g) for _ in range(4)]
+    adj_keep = (0, layout, n, batch, P(ts[0]), P(ts[1]), P(ts[2]), P(outs[0]), P(g)) + \
+        tuple(P(o) for o in adj_outs) + (h,)
+    T.check(L.tridiag_solve_fwd_bwd(*fb_args))
+    torch.cuda.synchronize()
+    # parity of the fused outputs against the FP64 oracle
+    a64, b64, c64, d64 = (v.astype(np.float64) for v in arrs)
+    xr = oracle.thomas(a64, b64, c64, d64, n, batch, layout=layout)
+    gr = oracle.adjoint(a64, b64, c64, xr, w.astype(np.float32).astype(np.float64), n, batch,
+                        layout=layout)
+    errs = [float(oracle.errstats(o.cpu().numpy().astype(np.float64), r, n, batch,
+                                  layout)["relerr"]) for o, r in zip(outs, (xr,) + tuple(gr))]
+    for _ in range(warmup):
+        L.tridiag_solve(*fwd_args)
+        L.tridiag_solve_fwd_bwd(*fb_args)
+        L.tridiag_adjoint(*adj_keep)
+    rc = [0]
+
+    def f1():
+        rc[0] |= L.tridiag_solve(*fwd_args)
+
+    def f2():
+        rc[0] |= L.tridiag_solve_fwd_bwd(*fb_args)
+
+    def f3():
+        rc[0] |= L.tridiag_adjoint(*adj_keep)
+    wf, ef = _event_time(torch, f1, steps, stream)
+    wb, eb = _event_time(torch, f2, steps, stream)
+    wa, ea = _event_time(torch, f3, steps, stream)
+    T.check(rc[0])
+    U = n * batch
+    r = {
+        "n": n, "batch": batch, "layout": {0: "contiguous", 2: "interleaved"}[layout],
+        "dtype": "f32",
+        "forward": {"kernel_ms": ef, "unknowns_per_s": U / (ef * 1e-3),
+                    "gbs": 5 * 4 * U / (ef * 1e-3) / 1e9},
+        "fused_fwd_bwd": {"kernel_ms": eb, "unknowns_per_s": U / (eb * 1e-3),
+                          "gbs": 10 * 4 * U / (eb * 1e-3) / 1e9},
+        "adjoint": {"kernel_ms": ea, "gbs": 9 * 4 * U / (ea * 1e-3) / 1e9},
+        "fwd_bwd_over_forward": eb / ef,
+        "relerr_x_ga_gb_gc_gd": errs,
+    }
+    log("fwd_bwd", json.dumps(r))
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +371,8 @@ def main():
     for name, dtname, layout in WORKLOADS:
         res[name] = run_workload(torch, T, name, dtname, layout, args.steps, args.warmup)
     main_r = res["f32_contiguous"]
+    ragged = run_ragged(torch, T, args.steps, args.warmup)
+    fwdbwd = run_fwd_bwd(torch, T, args.steps, args.warmup)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -198,6 +393,8 @@ def main():
         "workloads": {k: {kk: v[kk] for kk in ("unknowns_per_s", "ms_per_step", "kernel_ms",
                                                "layout", "plan", "parity", "roofline")}
                       for k, v in res.items()},
+        "ragged": ragged,
+        "fwd_bwd": fwdbwd,
     }
     print(json.dumps(line), flush=True)
 

@@ -137,9 +137,10 @@ int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_
  * System s holds rows [offsets[s], offsets[s+1]) of a, b, c, d (a_0 and c_{n-1}
  * of each system are ignored); x replaces d.  Sizes 0..8192 rows.  The systems
  * are solved by one persistent launch whose work list (size buckets, a tile
- * shape and an interface solver per bucket, largest bucket first) is planned on
- * the host and kept in `work`, a device buffer of tridiag_ragged_work_size(n_sys)
- * bytes.
+ * shape and an interface solver per bucket, largest bucket first; each item
+ * records the first row and size of its systems) is planned on the host from
+ * the host copy of the offsets and kept in `work`, a device buffer of
+ * tridiag_ragged_work_size(n_sys) bytes.
  *
  *   tridiag_ragged_plan   builds the work list from the host copy of the
  *                         offsets and uploads it (synchronises `stream`; do not
@@ -162,11 +163,10 @@ int tridiag_ragged_shape_info(int index, int *M, int *NWS, int *S, int64_t *cap)
 int tridiag_ragged_bucket(int precision, int64_t n, int *shape, int *iface);
 int tridiag_ragged_plan(int precision, int64_t n_sys, const int64_t *offsets_host, int shape,
                         int iface, void *work, void *stream);
-int tridiag_ragged_solve(int precision, int64_t n_sys, const int64_t *offsets_dev, void *a,
-                         void *b, void *c, void *d, void *work, void *stream);
-int tridiag_solve_ragged(int precision, int64_t n_sys, const int64_t *offsets_host,
-                         const int64_t *offsets_dev, void *a, void *b, void *c, void *d,
+int tridiag_ragged_solve(int precision, int64_t n_sys, void *a, void *b, void *c, void *d,
                          void *work, void *stream);
+int tridiag_solve_ragged(int precision, int64_t n_sys, const int64_t *offsets_host, void *a,
+                         void *b, void *c, void *d, void *work, void *stream);
 
 /* Adjoint of the solve.  Given a, b, c, the forward solution x and the upstream
  * gradient g = dL/dx (all layout `layout`, layouts 0 and 2), writes

@@ -64,7 +64,7 @@ def lib():
     L.tridiag_solve.restype = ci
     L.tridiag_ragged_work_size.argtypes = [i64]
     L.tridiag_ragged_work_size.restype = i64
-    L.tridiag_solve_ragged.argtypes = [ci, i64, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.tridiag_solve_ragged.argtypes = [ci, i64, vp, vp, vp, vp, vp, vp, vp]
     L.tridiag_solve_ragged.restype = ci
     L.tridiag_adjoint.argtypes = [ci, ci, i64, i64] + [vp] * 10
     L.tridiag_adjoint.restype = ci
@@ -91,7 +91,7 @@ def lib():
     L.tridiag_ragged_bucket.restype = ci
     L.tridiag_ragged_plan.argtypes = [ci, i64, vp, ci, ci, vp, vp]
     L.tridiag_ragged_plan.restype = ci
-    L.tridiag_ragged_solve.argtypes = [ci, i64, vp, vp, vp, vp, vp, vp, vp]
+    L.tridiag_ragged_solve.argtypes = [ci, i64, vp, vp, vp, vp, vp, vp]
     L.tridiag_ragged_solve.restype = ci
     _lib = L
     return L

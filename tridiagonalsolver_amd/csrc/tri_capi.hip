@@ -461,12 +461,12 @@ static void ragged_choice(int prec, int64_t n, int &shape, int &iface) {
 }
 
 static int64_t ragged_work_bytes(int64_t n_sys) {
-    const int64_t raw = kRaggedHeaderBytes + (int64_t)(4 * kRaggedItemInts + 4) * n_sys;
+    const int64_t raw = kRaggedHeaderBytes + (int64_t)kRaggedRecInts * 4 * n_sys;
     return (raw + 255) & ~(int64_t)255;
 }
 
-// Host planner: validates the offsets, buckets the systems, orders the work
-// items largest bucket first and copies the schedule into `work`.
+// Host planner: validates the offsets, buckets the systems, writes one record per
+// work item (largest bucket first) and copies the schedule into `work`.
 static int ragged_plan_impl(int prec, int64_t n_sys, const int64_t *off, int force_shape,
                             int force_iface, void *work, hipStream_t st) {
     if (prec != TRIDIAG_SINGLE && prec != TRIDIAG_DOUBLE)
@@ -480,8 +480,9 @@ static int ragged_plan_impl(int prec, int64_t n_sys, const int64_t *off, int for
         return fail(TRIDIAG_EINVAL, "interface solver id out of range");
     if (n_sys > 0 && off[0] < 0) return fail(TRIDIAG_EINVAL, "offsets[0] is negative");
     // (shape, iface) bucket of every system
+    constexpr int NF = kIfaceCR + 1;
     std::vector<int32_t> key((size_t)n_sys, -1);
-    std::vector<int64_t> cnt((size_t)kRaggedShapes * 5, 0);
+    std::vector<int64_t> cnt((size_t)kRaggedShapes * NF, 0);
     for (int64_t s = 0; s < n_sys; ++s) {
         const int64_t n = off[s + 1] - off[s];
         if (n < 0) return fail(TRIDIAG_EINVAL, "offsets must be non-decreasing");
@@ -499,65 +500,69 @@ static int ragged_plan_impl(int prec, int64_t n_sys, const int64_t *off, int for
             shape = force_shape;
         }
         if (force_iface >= 0) iface = force_iface;
-        key[(size_t)s] = shape * 5 + iface;
+        key[(size_t)s] = shape * NF + iface;
         ++cnt[(size_t)key[(size_t)s]];
     }
     // buckets in order of decreasing capacity (then decreasing systems per item)
     std::vector<int> order(cnt.size());
     for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
     std::stable_sort(order.begin(), order.end(), [](int x, int y) {
-        const int sx = x / 5, sy = y / 5;
+        const int sx = x / NF, sy = y / NF;
         if (ragged_cap(sx) != ragged_cap(sy)) return ragged_cap(sx) > ragged_cap(sy);
         return kRaggedShapeTab[sx].S < kRaggedShapeTab[sy].S;
     });
+    // systems of each bucket, in input order
     std::vector<int64_t> first(cnt.size(), 0);
     int64_t pos = 0;
     for (int k : order) {
         first[(size_t)k] = pos;
         pos += cnt[(size_t)k];
     }
-    const int64_t n_active = pos;
-    const int64_t bytes = ragged_work_bytes(n_sys);
-    std::vector<char> host((size_t)bytes, 0);
-    int32_t *items = reinterpret_cast<int32_t *>(host.data() + kRaggedHeaderBytes);
-    int32_t *perm = items + (size_t)kRaggedItemInts * n_sys;
+    std::vector<int32_t> perm((size_t)pos);
     {
         std::vector<int64_t> fill(first);
         for (int64_t s = 0; s < n_sys; ++s)
-            if (key[(size_t)s] >= 0) perm[fill[(size_t)key[(size_t)s]]++] = (int32_t)s;
+            if (key[(size_t)s] >= 0) perm[(size_t)fill[(size_t)key[(size_t)s]]++] = (int32_t)s;
     }
+    const int64_t bytes = ragged_work_bytes(n_sys);
+    std::vector<char> host((size_t)bytes, 0);
+    int32_t *recs = reinterpret_cast<int32_t *>(host.data() + kRaggedHeaderBytes);
     int64_t nitems = 0;
     for (int k : order) {
-        const int S = kRaggedShapeTab[k / 5].S;
+        const int S = kRaggedShapeTab[k / NF].S;
         for (int64_t i = 0; i < cnt[(size_t)k]; i += S) {
-            int32_t *w = items + (size_t)kRaggedItemInts * nitems++;
-            w[0] = k / 5;
-            w[1] = k % 5;
-            w[2] = (int32_t)(first[(size_t)k] + i);
-            w[3] = (int32_t)std::min<int64_t>(S, cnt[(size_t)k] - i);
+            int32_t *r = recs + (size_t)kRaggedRecInts * nitems++;
+            const int count = (int)std::min<int64_t>(S, cnt[(size_t)k] - i);
+            r[0] = k / NF;
+            r[1] = k % NF;
+            r[2] = count;
+            for (int q = 0; q < count; ++q) {
+                const int64_t s = perm[(size_t)(first[(size_t)k] + i + q)];
+                const uint64_t o = (uint64_t)off[s];
+                r[4 + 3 * q] = (int32_t)(uint32_t)(o & 0xffffffffu);
+                r[5 + 3 * q] = (int32_t)(uint32_t)(o >> 32);
+                r[6 + 3 * q] = (int32_t)(off[s + 1] - off[s]);
+            }
         }
     }
-    (void)n_active;
     RaggedHeader h{kRaggedMagic, n_sys, nitems, 0, prec};
     memcpy(host.data(), &h, sizeof h);
-    hipError_t e = hipMemcpyAsync(work, host.data(), (size_t)bytes, hipMemcpyHostToDevice, st);
+    const size_t used = (size_t)(kRaggedHeaderBytes + (int64_t)kRaggedRecInts * 4 * nitems);
+    hipError_t e = hipMemcpyAsync(work, host.data(), used, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "ragged plan upload");
     e = hipStreamSynchronize(st);
     return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "ragged plan upload");
 }
 
 template <typename T>
-static int ragged_solve_impl(int prec, int64_t n_sys, const int64_t *off_dev, T *a, T *b, T *c,
-                             T *d, void *work, hipStream_t st) {
+static int ragged_solve_impl(int prec, int64_t n_sys, T *a, T *b, T *c, T *d, void *work,
+                             hipStream_t st) {
     if (n_sys < 0 || n_sys > 0x7fffffff) return fail(TRIDIAG_EINVAL, "n_sys out of range");
     if (n_sys == 0) return TRIDIAG_OK;
-    if (!work || !off_dev || !a || !b || !c || !d) return fail(TRIDIAG_EINVAL, "null pointer");
+    if (!work || !a || !b || !c || !d) return fail(TRIDIAG_EINVAL, "null pointer");
     char *w = reinterpret_cast<char *>(work);
-    RaggedParams<T> p{a, b, c, d, off_dev, reinterpret_cast<RaggedHeader *>(w),
-                      reinterpret_cast<const int32_t *>(w + kRaggedHeaderBytes),
-                      reinterpret_cast<const int32_t *>(w + kRaggedHeaderBytes) +
-                          (size_t)kRaggedItemInts * n_sys,
-                      n_sys, prec};
+    RaggedParams<T> p{a, b, c, d, reinterpret_cast<RaggedHeader *>(w),
+                      reinterpret_cast<const int32_t *>(w + kRaggedHeaderBytes), n_sys, prec};
     hipError_t e;
     if constexpr (sizeof(T) == 4) e = launch_ragged_f32(p, ragged_grid_f32(), st);
     else e = launch_ragged_f64(p, ragged_grid_f64(), st);
@@ -744,25 +749,24 @@ int tridiag_ragged_plan(int precision, int64_t n_sys, const int64_t *offsets_hos
                             (hipStream_t)stream);
 }
 
-int tridiag_ragged_solve(int precision, int64_t n_sys, const int64_t *offsets_dev, void *a,
-                         void *b, void *c, void *d, void *work, void *stream) {
+int tridiag_ragged_solve(int precision, int64_t n_sys, void *a, void *b, void *c, void *d,
+                         void *work, void *stream) {
     g_last_error.clear();
     hipStream_t st = (hipStream_t)stream;
     if (precision == TRIDIAG_SINGLE)
-        return ragged_solve_impl<float>(precision, n_sys, offsets_dev, (float *)a, (float *)b,
-                                        (float *)c, (float *)d, work, st);
+        return ragged_solve_impl<float>(precision, n_sys, (float *)a, (float *)b, (float *)c,
+                                        (float *)d, work, st);
     if (precision == TRIDIAG_DOUBLE)
-        return ragged_solve_impl<double>(precision, n_sys, offsets_dev, (double *)a, (double *)b,
-                                         (double *)c, (double *)d, work, st);
+        return ragged_solve_impl<double>(precision, n_sys, (double *)a, (double *)b, (double *)c,
+                                         (double *)d, work, st);
     return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
 }
 
-int tridiag_solve_ragged(int precision, int64_t n_sys, const int64_t *offsets_host,
-                         const int64_t *offsets_dev, void *a, void *b, void *c, void *d,
-                         void *work, void *stream) {
+int tridiag_solve_ragged(int precision, int64_t n_sys, const int64_t *offsets_host, void *a,
+                         void *b, void *c, void *d, void *work, void *stream) {
     const int rc = tridiag_ragged_plan(precision, n_sys, offsets_host, -1, -1, work, stream);
     if (rc != TRIDIAG_OK) return rc;
-    return tridiag_ragged_solve(precision, n_sys, offsets_dev, a, b, c, d, work, stream);
+    return tridiag_ragged_solve(precision, n_sys, a, b, c, d, work, stream);
 }
 
 int tridiag_adjoint(int precision, int layout, int64_t n_eqt, int64_t n_batch, const void *a,

@@ -29,6 +29,12 @@ int ragged_grid_f64() { return ragged_grid<double>(); }
 
 template <typename T>
 static hipError_t launch_ragged(const RaggedParams<T> &p, int grid, hipStream_t st) {
+    static const hipError_t granted = [] {
+        return hipFuncSetAttribute(reinterpret_cast<const void *>(ragged_kernel<T>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   ragged_lds_bytes<T>());
+    }();
+    if (granted != hipSuccess) return granted;
     hipLaunchKernelGGL(ragged_reset_kernel, dim3(1), dim3(64), 0, st, p.hdr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

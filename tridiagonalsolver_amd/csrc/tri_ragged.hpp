@@ -6,9 +6,12 @@
 // that fits a 512-lane workgroup (S systems per workgroup, NWS waves = 64*NWS
 // lanes per system, M rows per lane) and an interface solver (Wang-Mou, PCR,
 // PCR + Thomas, CR).  A work item is one workgroup-load: up to S systems of one
-// bucket.  Items are ordered largest first; persistent workgroups take them
-// from an atomic counter until the list is exhausted, and every workgroup
-// reaches the exit once it draws an index past the end.
+// bucket, described by a 128-byte record (shape, interface solver and the first
+// row and size of each system).  Items are ordered largest first; persistent
+// workgroups take them from an atomic counter.  The index and record of the next
+// item are fetched while the current one is solved, so a workgroup starts the
+// coefficient loads of an item without a dependent chain of global reads.  Every
+// workgroup leaves the loop once it draws an index past the end of the list.
 #pragma once
 #include "tri_host.hpp"
 
@@ -24,11 +27,15 @@ constexpr RaggedShape kRaggedShapeTab[kRaggedShapes] = {
     {16, 2, 4}, {8, 8, 1},  {16, 4, 2}, {16, 8, 1}, {4, 2, 4},  {4, 4, 2}};
 constexpr int ragged_cap(int k) { return 64 * kRaggedShapeTab[k].NWS * kRaggedShapeTab[k].M; }
 constexpr int kRaggedLanes = 512;
+constexpr int kRaggedWaves = kRaggedLanes / 64;
 constexpr int kRaggedMaxN = 64 * 8 * 16;
+constexpr int kRaggedMaxM = 16;
 
-// Work buffer: header, then items[nitems][4] = {shape, interface solver, first
-// index into perm, number of systems}, then perm[n_sys] (system ids grouped by
-// bucket).  Written by the host planner once; the solve only resets `counter`.
+// Work buffer: a 64-byte header, then one 128-byte record per item:
+//   r[0] shape, r[1] interface solver, r[2] number of systems, r[3] 0,
+//   r[4 + 3k], r[5 + 3k]  first row of system slot k (int64, low word first),
+//   r[6 + 3k]             rows of system slot k.
+// Written by the host planner once; a solve only resets `counter`.
 struct RaggedHeader {
     int64_t magic;
     int64_t n_sys;
@@ -36,17 +43,16 @@ struct RaggedHeader {
     int32_t counter;     // work-queue head, zeroed before every solve
     int32_t precision;
 };
-constexpr int64_t kRaggedMagic = 0x52414747454431LL;
+constexpr int64_t kRaggedMagic = 0x52414747454432LL;
 constexpr int64_t kRaggedHeaderBytes = 64;
-constexpr int kRaggedItemInts = 4;
+constexpr int kRaggedRecInts = 32;
+constexpr int kRaggedMaxSlots = 8;
 
 template <typename T> struct RaggedParams {
     const T *a, *b, *c;
     T *d;
-    const int64_t *offsets;   // device, n_sys + 1
     RaggedHeader *hdr;
-    const int32_t *items;     // [nitems][4]
-    const int32_t *perm;      // [n_sys]
+    const int32_t *recs;      // [nitems][32]
     int64_t n_sys;
     int32_t precision;
 };
@@ -57,16 +63,55 @@ __device__ __forceinline__ int64_t wave_uniform(int64_t x) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// One item of shape (M, NWS, S): the lanes of wave w belong to system slot w / NWS.
-// Loads go through per-wave buffer descriptors based at the system's first row,
-// bounded by its last one, with 16-byte loads and stores wherever 16 bytes of
-// rows remain.
+// 16-byte chunk starting at element e (VV elements) of a system of n rows: the
+// whole chunk when it lies inside the system, its leading elements when it
+// straddles the end, zero past the end.  Never reads outside the system's rows.
+template <typename T>
+__device__ __forceinline__ u32x4 ragged_chunk_load(__amdgpu_buffer_rsrc_t r, int64_t e, int64_t n) {
+    constexpr int VV = 16 / (int)sizeof(T);
+    u32x4 t = {0u, 0u, 0u, 0u};
+    if (e + VV <= n) {
+        t = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(e * (int64_t)sizeof(T)), 0, cpol<true>());
+    } else if (e < n) {
+#pragma unroll
+        for (int v = 0; v < VV; ++v)
+            if (e + v < n)
+                reinterpret_cast<T *>(&t)[v] = bload<T, true>(r, (int)((e + v) * (int64_t)sizeof(T)), 0);
+    }
+    return t;
+}
+
+template <typename T>
+__device__ __forceinline__ void ragged_chunk_store(u32x4 t, __amdgpu_buffer_rsrc_t r, int64_t e,
+                                                   int64_t n) {
+    constexpr int VV = 16 / (int)sizeof(T);
+    if (e + VV <= n) {
+        __builtin_amdgcn_raw_buffer_store_b128(t, r, (int)(e * (int64_t)sizeof(T)), 0, cpol<true>());
+    } else if (e < n) {
+#pragma unroll
+        for (int v = 0; v < VV; ++v)
+            if (e + v < n)
+                bstore<T, true>(reinterpret_cast<const T *>(&t)[v], r,
+                                (int)((e + v) * (int64_t)sizeof(T)), 0);
+    }
+}
+
+// One item of shape (M, NWS, S): the lanes of wave w belong to system slot w / NWS
+// and wave w % NWS of that system owns rows [64*M*(w % NWS), 64*M*(w % NWS + 1)).
+// Segments of at least 16 bytes per lane are read as whole 16-byte chunks of
+// the wave's row block (consecutive lanes, consecutive chunks) and handed to
+// the lanes through LDS; shorter segments are read row by row.
 template <typename T, int M, int NWS, int S>
-__device__ __forceinline__ void ragged_item(const RaggedParams<T> &p, int iface, int start,
-                                            int count, char *dyn_lds) {
-    constexpr bool VEC = M >= Vec16<T>::N;
-    using Sh = TileShape<T, M, S * NWS, S, false, VEC, false, false, true, true, false>;
-    constexpr int SW = 64, CH = Sh::CH, V = Sh::V;
+__device__ __forceinline__ void ragged_item(const RaggedParams<T> &p, const int32_t *rec,
+                                            int iface, int count, u32x4 *xbuf,
+                                            char *iface_lds) {
+    constexpr int VV = 16 / (int)sizeof(T);
+    constexpr bool XPL = M * (int)sizeof(T) >= 16;          // whole-chunk loads
+    constexpr int NCH = XPL ? 64 * M / VV : 1;               // 16-byte chunks per wave block
+    constexpr int NQ = XPL ? NCH / 64 : 1;                   // chunks per lane
+    constexpr int PL = XPL ? M / VV : 1;                     // chunks per lane segment
+    using Sh = TileShape<T, M, S * NWS, S, false, false, false, false, true, true, XPL>;
+    constexpr int SW = 64, CH = Sh::CH;
     // Opaque copy of the lane id: without it the lane-dependent values of every
     // shape are hoisted out of the persistent loop and stay live in VGPRs across
     // all items of every shape.
@@ -77,34 +122,60 @@ __device__ __forceinline__ void ragged_item(const RaggedParams<T> &p, int iface,
     const int sl = wave / NWS, ws = wave % NWS, kw = lane;
     // slots past `count` are idle systems (identity rows) that still take part
     // in the workgroup-wide interface solve
-    bool valid = sl < count && (int64_t)start + sl < p.n_sys;
+    const bool valid = sl < count;
     int64_t off = 0, n = 0;
     if (valid) {
-        const int32_t sid = p.perm[start + sl];
-        if (sid >= 0 && sid < p.n_sys) {
-            off = p.offsets[sid];
-            n = p.offsets[sid + 1] - off;
-        }
-        if (n <= 0 || n > 64 * NWS * M) { valid = false; n = 0; off = 0; }
+        off = (int64_t)(((uint64_t)(uint32_t)rec[5 + 3 * sl] << 32) | (uint32_t)rec[4 + 3 * sl]);
+        n = rec[6 + 3 * sl];
     }
     off = wave_uniform(off);
     n = wave_uniform(n);
     const int64_t row0 = (int64_t)(ws * SW + kw) * M;
+    const int64_t wrow = (int64_t)ws * 64 * M;               // first row of the wave block
     const int32_t nb = (int32_t)(n * (int64_t)sizeof(T));
     const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.a + off, nb), rb = make_rsrc(p.b + off, nb),
                                  rc = make_rsrc(p.c + off, nb), rd = make_rsrc(p.d + off, nb);
-    const int vo = (int)(row0 * (int64_t)sizeof(T));
-    PredLoader<T, CH, V, true> ld{ra, rb, rc, rd, vo, (int)sizeof(T), row0, n};
     T al_r[Sh::NI];
     auto al_store = [&](int j, T v) { al_r[j] = v; };
     SweepOut<T, Sh::NI> o;
-    lane_sweep<T, Sh>(o, ld, row0, n, al_store);
+    if constexpr (XPL) {
+        const __amdgpu_buffer_rsrc_t rr[4] = {ra, rb, rc, rd};
+        u32x4 g[4][NQ];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+                g[k][q] = ragged_chunk_load<T>(rr[k], wrow + (int64_t)(q * 64 + lane) * VV, n);
+        T rows[4][M];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k > 0) wave_lds_sync();          // previous array fully read by all lanes
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) xbuf[q * 64 + lane] = g[k][q];
+            wave_lds_sync();                     // block written before lanes read their rows
+#pragma unroll
+            for (int i = 0; i < PL; ++i) {
+                const u32x4 t = xbuf[lane * PL + i];
+#pragma unroll
+                for (int v = 0; v < VV; ++v) rows[k][i * VV + v] = reinterpret_cast<const T *>(&t)[v];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (row0 + j >= n) { rows[0][j] = 0; rows[1][j] = 1; rows[2][j] = 0; rows[3][j] = 0; }
+        RegLoader<T, CH, M> ld{rows};
+        lane_sweep<T, Sh>(o, ld, row0, n, al_store);
+    } else {
+        PredLoader<T, CH, 1, true> ld{ra, rb, rc, rd, (int)(row0 * (int64_t)sizeof(T)),
+                                      (int)sizeof(T), row0, n};
+        lane_sweep<T, Sh>(o, ld, row0, n, al_store);
+    }
     const T DLr = o.bL - o.cL * o.SA, ELr = o.cL * o.PI, FLr = o.dL - o.cL * o.SD;
     const T ARr = -(o.aR * o.alp), DRr = o.bR - o.aR * o.g, FRr = o.dR - o.aR * o.del;
     T xL, xR;
     interface_solve<T, S, SW, NWS, 1, 64 * NWS, kRaggedLanes>(
-        o.aL, DLr, ELr, FLr, ARr, DRr, o.cR, FRr, tid, lane, sl, kw, ws, iface, 0, dyn_lds, xL, xR);
-    if (!valid || row0 >= n) return;
+        o.aL, DLr, ELr, FLr, ARr, DRr, o.cR, FRr, tid, lane, sl, kw, ws, iface, 0, iface_lds, xL, xR);
+    if (!valid || wrow >= n) return;                         // wave-uniform
     T xs[M];
     xs[0] = xL;
     xs[M - 1] = xR;
@@ -114,19 +185,34 @@ __device__ __forceinline__ void ragged_item(const RaggedParams<T> &p, int iface,
         xn = (o.de[j] - al_r[j] * xL) - o.ga[j] * xn;
         xs[j + 1] = xn;
     }
+    if constexpr (XPL) {
+        wave_lds_sync();
 #pragma unroll
-    for (int j0 = 0; j0 < M; j0 += V) {
-        if (V > 1 && row0 + j0 + V <= n) {
+        for (int i = 0; i < PL; ++i) {
             u32x4 t;
 #pragma unroll
-            for (int v = 0; v < V; ++v) reinterpret_cast<T *>(&t)[v] = xs[j0 + v];
-            __builtin_amdgcn_raw_buffer_store_b128(t, rd, vo, j0 * (int)sizeof(T), cpol<true>());
-        } else {
-#pragma unroll
-            for (int v = 0; v < V; ++v)
-                if (row0 + j0 + v < n) bstore<T, true>(xs[j0 + v], rd, vo, (j0 + v) * (int)sizeof(T));
+            for (int v = 0; v < VV; ++v) reinterpret_cast<T *>(&t)[v] = xs[i * VV + v];
+            xbuf[lane * PL + i] = t;
         }
+        wave_lds_sync();
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            ragged_chunk_store<T>(xbuf[q * 64 + lane], rd, wrow + (int64_t)(q * 64 + lane) * VV, n);
+    } else {
+        const int vo = (int)(row0 * (int64_t)sizeof(T));
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (row0 + j < n) bstore<T, true>(xs[j], rd, vo, j * (int)sizeof(T));
     }
+}
+
+// Dynamic LDS of the persistent kernel: the per-wave transposition blocks, then
+// room for the LDS interface solvers.
+template <typename T> constexpr int ragged_xbuf_bytes() {
+    return kRaggedWaves * 64 * kRaggedMaxM * (int)sizeof(T);
+}
+template <typename T> constexpr int ragged_lds_bytes() {
+    return ragged_xbuf_bytes<T>() + iface_lds_bytes<T>(kRaggedLanes);
 }
 
 // Persistent kernel.  A work buffer that does not carry this plan's magic,
@@ -134,28 +220,53 @@ __device__ __forceinline__ void ragged_item(const RaggedParams<T> &p, int iface,
 template <typename T>
 __global__ void __launch_bounds__(kRaggedLanes) ragged_kernel(RaggedParams<T> p) {
     extern __shared__ __attribute__((aligned(16))) char tri_dyn_lds[];
-    __shared__ int s_item;
+    __shared__ int32_t s_rec[2][kRaggedRecInts];
+    __shared__ int32_t s_idx[2];
     const RaggedHeader *h = p.hdr;
     if (h->magic != kRaggedMagic || h->n_sys != p.n_sys || h->precision != p.precision) return;
-    const int64_t nitems = h->nitems;
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(&p.hdr->counter, 1);
-        __syncthreads();
-        // the item index and its fields are workgroup-uniform: keep them in SGPRs
-        // so that the shape switch is a scalar branch
-        const int it = __builtin_amdgcn_readfirstlane(s_item);
-        __syncthreads();
+    const int nitems = (int)h->nitems;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    u32x4 *xbuf = reinterpret_cast<u32x4 *>(tri_dyn_lds) +
+                  (size_t)wave * (64 * kRaggedMaxM * sizeof(T) / 16);
+    char *iface_lds = tri_dyn_lds + ragged_xbuf_bytes<T>();
+    // Wave 0 keeps the queue two items ahead: the record of the next item and
+    // the index of the one after it are in flight while an item is solved.
+    int nxt = 0;
+    if (wave == 0) {
+        int i0 = 0, i1 = 0;
+        if (lane == 0) {
+            i0 = atomicAdd(&p.hdr->counter, 1);
+            i1 = atomicAdd(&p.hdr->counter, 1);
+        }
+        i0 = __builtin_amdgcn_readfirstlane(i0);
+        i1 = __builtin_amdgcn_readfirstlane(i1);
+        if (lane < kRaggedRecInts)
+            s_rec[0][lane] = (i0 < nitems) ? p.recs[(int64_t)i0 * kRaggedRecInts + lane] : 0;
+        if (lane == 0) s_idx[0] = i0;
+        nxt = i1;
+    }
+    __syncthreads();
+    for (int k = 0;; ++k) {
+        const int buf = k & 1;
+        const int it = __builtin_amdgcn_readfirstlane(s_idx[buf]);
         if (it < 0 || it >= nitems) break;   // every workgroup leaves once the list is empty
-        const int32_t *w = p.items + (int64_t)kRaggedItemInts * it;
-        const int shape = __builtin_amdgcn_readfirstlane(w[0]);
-        const int iface = __builtin_amdgcn_readfirstlane(w[1]);
-        const int start = __builtin_amdgcn_readfirstlane(w[2]);
-        const int count = __builtin_amdgcn_readfirstlane(w[3]);
+        // wave 0: start fetching the next record and the index after it
+        int32_t rnext = 0;
+        int nn = 0;
+        if (wave == 0) {
+            if (nxt < nitems && lane < kRaggedRecInts)
+                rnext = p.recs[(int64_t)nxt * kRaggedRecInts + lane];
+            if (lane == 0 && nxt < nitems) nn = atomicAdd(&p.hdr->counter, 1);
+        }
+        const int32_t *rec = s_rec[buf];
+        const int shape = __builtin_amdgcn_readfirstlane(rec[0]);
+        const int iface = __builtin_amdgcn_readfirstlane(rec[1]);
+        const int count = __builtin_amdgcn_readfirstlane(rec[2]);
         switch (shape) {
 #define TRI_RAGGED_CASE(K)                                                                   \
     case K:                                                                                \
         ragged_item<T, kRaggedShapeTab[K].M, kRaggedShapeTab[K].NWS, kRaggedShapeTab[K].S>( \
-            p, iface, start, count, tri_dyn_lds);                                          \
+            p, rec, iface, count, xbuf, iface_lds);                                        \
         break;
             TRI_RAGGED_CASE(0) TRI_RAGGED_CASE(1) TRI_RAGGED_CASE(2) TRI_RAGGED_CASE(3)
             TRI_RAGGED_CASE(4) TRI_RAGGED_CASE(5) TRI_RAGGED_CASE(6) TRI_RAGGED_CASE(7)
@@ -163,12 +274,15 @@ __global__ void __launch_bounds__(kRaggedLanes) ragged_kernel(RaggedParams<T> p)
 #undef TRI_RAGGED_CASE
         default: break;
         }
-        __syncthreads();                     // LDS of the item is free for the next one
+        if (wave == 0) {
+            if (lane < kRaggedRecInts) s_rec[buf ^ 1][lane] = rnext;
+            if (lane == 0) s_idx[buf ^ 1] = nxt;
+            nn = __builtin_amdgcn_readfirstlane(nn);
+            nxt = (nxt < nitems) ? nn : nxt;
+        }
+        __syncthreads();                     // next record visible; LDS of the item free
     }
 }
-
-// Dynamic LDS of the persistent kernel: room for the LDS interface solvers.
-template <typename T> constexpr int ragged_lds_bytes() { return iface_lds_bytes<T>(kRaggedLanes); }
 
 hipError_t launch_ragged_f32(const RaggedParams<float> &p, int grid, hipStream_t st);
 hipError_t launch_ragged_f64(const RaggedParams<double> &p, int grid, hipStream_t st);

@@ -80,7 +80,6 @@ class RaggedPlan:
             raise ValueError("offsets needs at least one entry")
         device = torch.device(device if device is not None else "cuda")
         self.device = device
-        self.offsets = self.offsets_host.to(device)
         self.work = torch.empty(int(lib().tridiag_ragged_work_size(self.n_sys)),
                                 dtype=torch.uint8, device=device)
         check(lib().tridiag_ragged_plan(self.precision, self.n_sys,
@@ -99,9 +98,8 @@ class RaggedPlan:
         for t in (a, b, c, d):
             if t.numel() < self.rows:
                 raise ValueError("tensors hold fewer rows than offsets[-1]")
-        check(lib().tridiag_ragged_solve(self.precision, self.n_sys, _ptr(self.offsets), _ptr(a),
-                                         _ptr(b), _ptr(c), _ptr(d), _ptr(self.work),
-                                         _stream_handle(a.device)))
+        check(lib().tridiag_ragged_solve(self.precision, self.n_sys, _ptr(a), _ptr(b), _ptr(c),
+                                         _ptr(d), _ptr(self.work), _stream_handle(a.device)))
         return d
 
 
