@@ -36,10 +36,14 @@ def _to(v):
 
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("layout", [0, 2])
-@pytest.mark.parametrize("n", [2, 7, 64, 512, 1000])
+@pytest.mark.parametrize("n", [2, 3, 7, 64, 255, 257, 512, 1000, 2049, 4096, 8191])
 def test_adjoint_and_fused(prec, layout, n):
     import torch
     import tridiagonalsolver_amd as T
+    if prec == 1 and n > 4096:
+        pytest.skip("FP64 adjoint tiles hold up to 4096 rows")
+    if layout == 2 and n > 1024:
+        pytest.skip("interleaved adjoint tiles hold up to 1024 rows")
     batch = 37
     dtype = DT[prec]
     announce("adjoint", prec, layout, n)

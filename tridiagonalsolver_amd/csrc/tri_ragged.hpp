@@ -57,45 +57,6 @@ template <typename T> struct RaggedParams {
     int32_t precision;
 };
 
-__device__ __forceinline__ int64_t wave_uniform(int64_t x) {
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
-// 16-byte chunk starting at element e (VV elements) of a system of n rows: the
-// whole chunk when it lies inside the system, its leading elements when it
-// straddles the end, zero past the end.  Never reads outside the system's rows.
-template <typename T>
-__device__ __forceinline__ u32x4 ragged_chunk_load(__amdgpu_buffer_rsrc_t r, int64_t e, int64_t n) {
-    constexpr int VV = 16 / (int)sizeof(T);
-    u32x4 t = {0u, 0u, 0u, 0u};
-    if (e + VV <= n) {
-        t = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(e * (int64_t)sizeof(T)), 0, cpol<true>());
-    } else if (e < n) {
-#pragma unroll
-        for (int v = 0; v < VV; ++v)
-            if (e + v < n)
-                reinterpret_cast<T *>(&t)[v] = bload<T, true>(r, (int)((e + v) * (int64_t)sizeof(T)), 0);
-    }
-    return t;
-}
-
-template <typename T>
-__device__ __forceinline__ void ragged_chunk_store(u32x4 t, __amdgpu_buffer_rsrc_t r, int64_t e,
-                                                   int64_t n) {
-    constexpr int VV = 16 / (int)sizeof(T);
-    if (e + VV <= n) {
-        __builtin_amdgcn_raw_buffer_store_b128(t, r, (int)(e * (int64_t)sizeof(T)), 0, cpol<true>());
-    } else if (e < n) {
-#pragma unroll
-        for (int v = 0; v < VV; ++v)
-            if (e + v < n)
-                bstore<T, true>(reinterpret_cast<const T *>(&t)[v], r,
-                                (int)((e + v) * (int64_t)sizeof(T)), 0);
-    }
-}
-
 // One item of shape (M, NWS, S): the lanes of wave w belong to system slot w / NWS
 // and wave w % NWS of that system owns rows [64*M*(w % NWS), 64*M*(w % NWS + 1)).
 // Segments of at least 16 bytes per lane are read as whole 16-byte chunks of
@@ -145,7 +106,7 @@ __device__ __forceinline__ void ragged_item(const RaggedParams<T> &p, const int3
         for (int k = 0; k < 4; ++k)
 #pragma unroll
             for (int q = 0; q < NQ; ++q)
-                g[k][q] = ragged_chunk_load<T>(rr[k], wrow + (int64_t)(q * 64 + lane) * VV, n);
+                g[k][q] = tail_chunk_load<T, true>(rr[k], wrow + (int64_t)(q * 64 + lane) * VV, n);
         T rows[4][M];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -197,7 +158,7 @@ __device__ __forceinline__ void ragged_item(const RaggedParams<T> &p, const int3
         wave_lds_sync();
 #pragma unroll
         for (int q = 0; q < NQ; ++q)
-            ragged_chunk_store<T>(xbuf[q * 64 + lane], rd, wrow + (int64_t)(q * 64 + lane) * VV, n);
+            tail_chunk_store<T, true>(xbuf[q * 64 + lane], rd, wrow + (int64_t)(q * 64 + lane) * VV, n);
     } else {
         const int vo = (int)(row0 * (int64_t)sizeof(T));
 #pragma unroll
