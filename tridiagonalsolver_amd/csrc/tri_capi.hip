@@ -17,6 +17,7 @@
 #include "../../include/tridiagonal.h"
 #include "tri_adjoint.hpp"
 #include "tri_host.hpp"
+#include "tri_large.hpp"
 #include "tri_ragged.hpp"
 
 namespace tri {
@@ -254,6 +255,71 @@ template <typename T> static int32_t buffer_extent(int64_t n, int64_t batch, int
     return bytes < (int64_t)0x7fffff00 ? (int32_t)bytes : 0;
 }
 
+// Largest n any on-chip tile of the contiguous layouts holds.
+template <typename T> static int64_t max_onchip_rows() {
+    static const int64_t v = [] {
+        TileTable<T> tab = tile_table<T>();
+        int64_t m = 0;
+        for (int i = 0; i < tab.count; ++i)
+            if (!tab.entries[i].IL && tab.entries[i].PAD) m = std::max<int64_t>(m, tab.entries[i].cap);
+        return m;
+    }();
+    return v;
+}
+
+// Bytes of input (four arrays) that one group of the chunked solve covers: small
+// enough that pass 3 rereads its chunks from the last-level cache.
+static int64_t large_group_bytes() {
+    static const int64_t v = [] {
+        if (const char *e = getenv("TRIDIAG_LARGE_GROUP_BYTES")) {
+            const long long x = atoll(e);
+            if (x > 0) return (int64_t)x;
+        }
+        return (int64_t)128 << 20;
+    }();
+    return v;
+}
+
+// Systems above the on-chip capacity: the three-pass chunked solve, in groups
+// of systems, with stream-ordered scratch for the chunk outer forms.
+template <typename T>
+static int large_impl(int algo, int64_t n, int64_t batch, int64_t sstride, T *a, T *b, T *c, T *d,
+                      hipStream_t st) {
+    const int prec = sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE;
+    const int shape = 0;
+    const int64_t C = large_chunk_rows(prec, shape);
+    const int64_t K = (n + C - 1) / C;
+    if (K > 1024)
+        return fail(TRIDIAG_EUNSUPPORTED, "n_eqt=" + std::to_string(n) + " needs more than 1024 "
+                                          "chunks of " + std::to_string(C) + " rows");
+    const int64_t per_sys = 4 * n * (int64_t)sizeof(T);
+    const int64_t G = std::max<int64_t>(1, std::min<int64_t>({batch, 65535,
+                                                              large_group_bytes() / per_sys}));
+    void *scratch = nullptr;
+    const size_t bytes = (size_t)(G * K * 8) * sizeof(T);
+    hipError_t e = hipMallocAsync(&scratch, bytes, st);
+    if (e != hipSuccess) return hip_fail(e, "chunked solve scratch");
+    const AlgoSpec sp = algo_spec(algo);
+    int rc = TRIDIAG_OK;
+    for (int64_t g0 = 0; g0 < batch && rc == TRIDIAG_OK; g0 += G) {
+        const int64_t cnt = std::min(G, batch - g0);
+        const int64_t o = g0 * sstride;
+        T *red = reinterpret_cast<T *>(scratch);
+        LargeParams<T> p{a + o, b + o, c + o, d + o, n, cnt, sstride, (int32_t)K, sp.iface,
+                         red, red + G * K * 6};
+        if constexpr (sizeof(T) == 4) e = launch_large_f32(p, shape, st);
+        else e = launch_large_f64(p, shape, st);
+        if (e == hipErrorInvalidConfiguration)
+            rc = fail(TRIDIAG_EUNSUPPORTED, "interface solver of algorithm " + std::to_string(algo) +
+                                                " needs more LDS than the chunk kernel leaves");
+        else if (e != hipSuccess)
+            rc = hip_fail(e, "chunked solve launch");
+    }
+    hipError_t ef = hipFreeAsync(scratch, st);
+    if (rc == TRIDIAG_OK && ef != hipSuccess) rc = hip_fail(ef, "chunked solve scratch");
+    return rc;
+}
+
 template <typename T>
 static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b, T *c, T *d,
                       hipStream_t st) {
@@ -304,6 +370,8 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
                         aligned16<T>(b) && aligned16<T>(c) && aligned16<T>(d);
     const int32_t bb = buffer_extent<T>(n, batch, sstride, layout);
     const TileEntry<T> *e = choose_tile<T>(algo, layout, n, vec_ok, bb == 0);
+    if (!e && layout != kInterleaved && n > max_onchip_rows<T>())
+        return large_impl<T>(algo, n, batch, sstride, a, b, c, d, st);
     if (!e) {
         return fail(TRIDIAG_EUNSUPPORTED,
                     "n_eqt=" + std::to_string(n) + " exceeds the on-chip capacity of algorithm " +

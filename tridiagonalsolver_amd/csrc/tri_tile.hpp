@@ -343,6 +343,56 @@ __device__ __forceinline__ void lane_sweep(SweepOut<T, Sh::NI> &o, const Loader 
     }
 }
 
+// Outer form (first row oL, last row oR) of a lane segment from its reduced rows
+//   L: aL u + DLr x_L + ELr x_R = FLr ;  R: ARr x_L + DRr x_R + cR v = FRr.
+template <typename T>
+__device__ __forceinline__ void segment_outer(T aL, T DLr, T ELr, T FLr, T ARr, T DRr, T cR,
+                                              T FRr, Outer<T> &oL, Outer<T> &oR) {
+    const T inv = rcp(DLr * DRr - ELr * ARr);
+    oL.r = (DRr * FLr - ELr * FRr) * inv;
+    oL.p = (DRr * aL) * inv;
+    oL.q = -(ELr * cR) * inv;
+    oR.r = (DLr * FRr - ARr * FLr) * inv;
+    oR.p = -(ARr * aL) * inv;
+    oR.q = (DLr * cR) * inv;
+}
+
+// Wang-Mou merges of the SW segments of a system inside one wave (segment kw of
+// system slot sl sits in lane sl % LS + LS * kw) into blocks of BS segments:
+// afterwards oL / oR are in outer form relative to the block's neighbours.
+// lds_x exchanges through the wave's 6*64-element LDS area xs instead of
+// ds_bpermute.
+template <typename T, int SW, int LS>
+__device__ __forceinline__ void wm_wave_merges(Outer<T> &oL, Outer<T> &oR, int lane, int sl,
+                                               int kw, int BS, bool lds_x, T *xs) {
+#pragma unroll
+    for (int t = 0; (1 << t) < SW; ++t) {
+        if ((1 << t) >= BS) break;
+        const int h = 1 << t;
+        const int blk = kw >> t;
+        const int pb = (blk & ~1) << t;              // first segment of the merged pair
+        const int srcA = sl % LS + LS * (pb + h - 1); // lane with A's last row
+        const int srcB = sl % LS + LS * (pb + h);     // lane with B's first row
+        T rA, pA, qA, rB, pB, qB;
+        if (lds_x) {
+            xs[lane * 6 + 0] = oL.r; xs[lane * 6 + 1] = oL.p; xs[lane * 6 + 2] = oL.q;
+            xs[lane * 6 + 3] = oR.r; xs[lane * 6 + 4] = oR.p; xs[lane * 6 + 5] = oR.q;
+            wave_lds_sync();
+            rA = xs[srcA * 6 + 3]; pA = xs[srcA * 6 + 4]; qA = xs[srcA * 6 + 5];
+            rB = xs[srcB * 6 + 0]; pB = xs[srcB * 6 + 1]; qB = xs[srcB * 6 + 2];
+            wave_lds_sync();
+        } else {
+            rA = lane_read(oR.r, srcA); pA = lane_read(oR.p, srcA); qA = lane_read(oR.q, srcA);
+            rB = lane_read(oL.r, srcB); pB = lane_read(oL.p, srcB); qB = lane_read(oL.q, srcB);
+        }
+        T RB, PB, QB, RA, PA, QA;
+        wm_pair_coeffs(rA, pA, qA, rB, pB, qB, RB, PB, QB, RA, PA, QA);
+        const bool inA = (blk & 1) == 0;
+        wm_merge_row(oL, inA, RB, PB, QB, RA, PA, QA);
+        wm_merge_row(oR, inA, RB, PB, QB, RA, PA, QA);
+    }
+}
+
 // Solves the interface system of a workgroup of tiles from every lane's reduced
 // rows   L: aL u + DLr x_L + ELr x_R = FLr ;  R: ARr x_L + DRr x_R + cR v = FRr
 // and returns x at the lane's first (xL) and last (xR) rows.  Every lane of the
@@ -383,45 +433,11 @@ __device__ __forceinline__ void interface_solve(T aL, T DLr, T ELr, T FLr, T ARr
     }
     // Wang-Mou: outer form of the segment, merges across lanes, pGE chain across blocks
     Outer<T> oL, oR;
-    {
-        T inv = rcp(DLr * DRr - ELr * ARr);
-        oL.r = (DRr * FLr - ELr * FRr) * inv;
-        oL.p = (DRr * aL) * inv;
-        oL.q = -(ELr * cR) * inv;
-        oR.r = (DLr * FRr - ARr * FLr) * inv;
-        oR.p = -(ARr * aL) * inv;
-        oR.q = (DLr * cR) * inv;
-    }
+    segment_outer(aL, DLr, ELr, FLr, ARr, DRr, cR, FRr, oL, oR);
     const int half = iface == kIfaceWMHalf ? 1 : 0;
     const int BS = SW >> half;                   // segments per merged block
-    const bool lds_x = xchg_lds != 0;            // exchange through LDS instead of bpermute
     T *xs = reinterpret_cast<T *>(tri_dyn_lds) + (size_t)(tid & ~63) * 6;
-#pragma unroll
-    for (int t = 0; (1 << t) < SW; ++t) {
-        if ((1 << t) >= BS) break;
-        const int h = 1 << t;
-        const int blk = kw >> t;
-        const int pb = (blk & ~1) << t;              // first segment of the merged pair
-        const int srcA = sl % LS + LS * (pb + h - 1); // lane with A's last row
-        const int srcB = sl % LS + LS * (pb + h);     // lane with B's first row
-        T rA, pA, qA, rB, pB, qB;
-        if (lds_x) {
-            xs[lane * 6 + 0] = oL.r; xs[lane * 6 + 1] = oL.p; xs[lane * 6 + 2] = oL.q;
-            xs[lane * 6 + 3] = oR.r; xs[lane * 6 + 4] = oR.p; xs[lane * 6 + 5] = oR.q;
-            wave_lds_sync();
-            rA = xs[srcA * 6 + 3]; pA = xs[srcA * 6 + 4]; qA = xs[srcA * 6 + 5];
-            rB = xs[srcB * 6 + 0]; pB = xs[srcB * 6 + 1]; qB = xs[srcB * 6 + 2];
-            wave_lds_sync();
-        } else {
-            rA = lane_read(oR.r, srcA); pA = lane_read(oR.p, srcA); qA = lane_read(oR.q, srcA);
-            rB = lane_read(oL.r, srcB); pB = lane_read(oL.p, srcB); qB = lane_read(oL.q, srcB);
-        }
-        T RB, PB, QB, RA, PA, QA;
-        wm_pair_coeffs(rA, pA, qA, rB, pB, qB, RB, PB, QB, RA, PA, QA);
-        const bool inA = (blk & 1) == 0;
-        wm_merge_row(oL, inA, RB, PB, QB, RA, PA, QA);
-        wm_merge_row(oR, inA, RB, PB, QB, RA, PA, QA);
-    }
+    wm_wave_merges<T, SW, LS>(oL, oR, lane, sl, kw, BS, xchg_lds != 0, xs);
 
     // pGE chain over the NB = NWS << half merged blocks of each system
     T u = 0, v = 0;
