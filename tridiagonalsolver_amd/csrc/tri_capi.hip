@@ -267,15 +267,18 @@ template <typename T> static int64_t max_onchip_rows() {
     return v;
 }
 
-// Bytes of input (four arrays) that one group of the chunked solve covers: small
-// enough that pass 3 rereads its chunks from the last-level cache.
+// Bytes of input (four arrays) that one group of the chunked solve covers
+// ($TRIDIAG_LARGE_GROUP_BYTES overrides).  Measured on MI355X at 2^20 x 16 and
+// 2^16 x 256 FP64: groups small enough for pass 3 to reread its chunks from the
+// 256 MB last-level cache (64-256 MB) ran 10-40% slower than one group, so the
+// default covers 1 GiB per group.
 static int64_t large_group_bytes() {
     static const int64_t v = [] {
         if (const char *e = getenv("TRIDIAG_LARGE_GROUP_BYTES")) {
             const long long x = atoll(e);
             if (x > 0) return (int64_t)x;
         }
-        return (int64_t)128 << 20;
+        return (int64_t)1 << 30;
     }();
     return v;
 }

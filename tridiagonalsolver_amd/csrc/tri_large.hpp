@@ -14,9 +14,8 @@
 //     folds the known boundary values into its first and last right-hand sides
 //     (d_F -= a_F u_k, d_E -= c_E v_k) and solves the chunk as an independent
 //     system with the tile solver.
-// HBM traffic: 4 words per row in pass 1, 4 + 1 in pass 3 (the chunk reread can
-// be served by the last-level cache when a group of systems fits in it), plus
-// 8 words per chunk of scratch.
+// HBM traffic: 4 words per row in pass 1, 4 + 1 in pass 3, plus 8 words per
+// chunk of scratch: 9 words per row against the 5 of an on-chip solve.
 #pragma once
 #include "tri_host.hpp"
 
@@ -61,13 +60,16 @@ __global__ void __launch_bounds__(64 * NW) chunk_kernel(LargeParams<T> p) {
     const int64_t row0 = (int64_t)(wave * 64 + lane) * M;   // chunk-relative first row
     u32x4 *xb = xbuf[wave];
 
-    // whole 16-byte chunks of the wave block, handed to the lanes through LDS
+    // whole 16-byte chunks of the wave block, handed to the lanes through LDS;
+    // pass 1 reads with the default cache policy so that pass 3 can find the
+    // chunk in the last-level cache, pass 3 streams
     u32x4 g[4][NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int a = 0; a < 4; ++a)
-            g[a][q] = tail_chunk_load<T, true>(rr[a], wrow + (int64_t)(q * 64 + lane) * VV, len);
+            g[a][q] = tail_chunk_load<T, PASS == kChunkSolve>(
+                rr[a], wrow + (int64_t)(q * 64 + lane) * VV, len);
     T rows[4][M];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
