@@ -277,7 +277,8 @@ def run_ragged(torch, T, steps, warmup, n_sys=65536, nmin=32, nmax=8192):
         "speedup_vs_plain_graph": wall_pg / wall_rg,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic("ragged_f32")},
+                     "traffic": (round(load_traffic("ragged_f32_traffic_per_alg_byte") * alg_bytes)
+                                 if load_traffic("ragged_f32_traffic_per_alg_byte") else None)},
     }
     log("ragged", json.dumps(r))
     return r
