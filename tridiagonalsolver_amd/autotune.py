@@ -65,6 +65,9 @@ def sweep(prec, layout, n, batch, reps=20, warmup=3, nsets=3, seed=1):
     L = _capi.lib()
     tiles = [t for t in _capi.tiles(prec) if t["interleaved"] == (layout == 2)
              and t["cap"] >= n and not t["wide_addr"] and t["padded"] == (t["cap"] != n)]
+    if not tiles:
+        _log("no on-chip kernel for prec %d layout %d n %d" % (prec, layout, n))
+        return []
     sets = _inputs(torch, prec, layout, n, batch, seed, nsets)
     xref = _reference(sets, prec, layout, n, batch)
     pristine = [s[3].clone() for s in sets]
