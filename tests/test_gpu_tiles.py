@@ -1,0 +1,62 @@
+"""Every compiled on-chip kernel configuration against the FP64 oracle.
+
+Each tile (rows per lane M, waves NW, systems per workgroup S, layout and load
+variant) is run through tridiag_solve_tile at its capacity, and padded tiles also
+at a size that leaves part of the last lane segment empty; the batch leaves the
+last workgroup partly idle.  Tolerance: normwise relative error <= 1e-5 (FP32) /
+<= 1e-12 (FP64) on every system.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from gpu_util import TOL, announce, check_against_oracle, make_case
+
+pytestmark = pytest.mark.gpu
+
+DT = {0: np.float32, 1: np.float64}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def _run_tile(prec, t, n, batch, seed):
+    import torch
+    import tridiagonalsolver_amd as T
+    layout = 2 if t["interleaved"] else 0
+    arrs = make_case(n, batch, layout, DT[prec], seed)
+    ts = [torch.from_numpy(np.ascontiguousarray(v)).cuda() for v in arrs]
+    rc = T.lib().tridiag_solve_tile(prec, t["index"], layout, n, batch,
+                                    *(ctypes.c_void_p(x.data_ptr()) for x in ts),
+                                    ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    T.check(rc)
+    torch.cuda.synchronize()
+    x = ts[3].cpu().numpy()
+    return check_against_oracle(arrs, x, n, batch, layout, DT[prec])
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("interleaved", [False, True])
+def test_every_tile(prec, interleaved):
+    import tridiagonalsolver_amd as T
+    fails = []
+    tiles = [t for t in T.tiles(prec) if t["interleaved"] == interleaved and not t["wide_addr"]]
+    assert tiles
+    for t in tiles:
+        sizes = [t["cap"]] if not t["padded"] else sorted({max(1, t["cap"] - t["M"] // 2 - 1),
+                                                            max(1, t["cap"] // 2 + 3)})
+        if t["vec"]:                     # 16-byte row loads need n a multiple of 16 bytes
+            v = 4 if prec == 0 else 2
+            sizes = sorted({max(v, k - k % v) for k in sizes})
+        batch = 2 * t["S"] + 1
+        for n in sizes:
+            announce("tile", prec, t, "n", n, "batch", batch)
+            st, res = _run_tile(prec, t, n, batch, seed=t["index"] * 7 + n)
+            if not (st["nonfinite"] == 0 and st["relerr"] <= TOL[DT[prec]]):
+                fails.append((t, n, st))
+    assert not fails, fails[:5]

@@ -56,6 +56,7 @@ C_SHAPES = {
 }
 ALDS_MIN_M = {"float": 32, "double": 16}
 VEC_W = {"float": 4, "double": 2}
+WORD = {"float": 4, "double": 8}
 VEC_W_BYTES = {"float": 16, "double": 16}
 # Interleaved shapes also built with 64-bit addressing, for arrays of 2 GiB or
 # more (contiguous batches are split into chunks below 2 GiB instead).
@@ -76,6 +77,9 @@ def _entries(T):
             out.append((M, NW, S, True, False, alds, False, True, False, False))
         if (M, NW) in BIG_IL:
             out.append((M, NW, S, True, False, False, True, True, False, False))
+        if S * WORD[T] in (64, 128) and not heavy:
+            # exact tiles reading whole row pieces, handed to lanes through LDS
+            out.append((M, NW, S, True, False, False, False, False, True, True))
     for (M, NW, S) in C_SHAPES[T]:
         heavy = M >= ALDS_MIN_M[T]
         alds = heavy and (NW == 16 or (T == "float" and M == 64) or (T == "double" and M == 32))

@@ -501,7 +501,8 @@ template <typename T, int M, int NW, int S, bool IL, bool VEC, bool ALDS, bool B
           bool NT, bool XP>
 __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
     using Sh = TileShape<T, M, NW, S, IL, VEC, ALDS, BIG, PAD, NT, XP>;
-    static_assert(!XP || (!IL && VEC && !PAD && !BIG), "transposed loads need exact contiguous tiles");
+    static_assert(!XP || (!PAD && !BIG && (IL ? (S * (int)sizeof(T)) % 16 == 0 : VEC)),
+                  "transposed loads need exact tiles (and 16-byte row pieces when interleaved)");
     constexpr int SW = Sh::SW, NWS = Sh::NWS, LS = Sh::LS, V = Sh::V, CH = Sh::CH;
     constexpr int LANES = 64 * NW;
     const int tid = threadIdx.x;
@@ -524,6 +525,8 @@ __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
     // Interior coefficients; al lives in LDS when ALDS (it is only read back in
     // the substitution), laid out [row][lane] so a wave's accesses are contiguous.
     __shared__ T sh_al[ALDS ? Sh::NI : 1][ALDS ? LANES : 1];
+    // transposition block of a wave: contiguous tiles 64 segments x M rows; interleaved
+    // tiles SW*M rows x S systems (the same element count)
     __shared__ u32x4 xbuf[XP ? NW : 1][XP ? 64 * M * (int)sizeof(T) / 16 : 1];
     (void)xbuf;
     extern __shared__ __attribute__((aligned(16))) char tri_dyn_lds[];
@@ -546,7 +549,42 @@ __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
     } else {
         __amdgpu_buffer_rsrc_t ra = make_rsrc(prm.a, nb), rb = make_rsrc(prm.b, nb),
                                rc = make_rsrc(prm.c, nb);
-        if constexpr (XP) {
+        if constexpr (XP && IL) {
+            // wave block: rows [ws*SW*M, (ws+1)*SW*M) of the tile's S systems; each row
+            // piece is S*sizeof(T) contiguous bytes = CPR 16-byte chunks, read by
+            // consecutive lanes; lane (sl, kw) then takes column sl of its M rows
+            constexpr int VV = 16 / (int)sizeof(T);
+            constexpr int CPR = S * (int)sizeof(T) / 16;
+            constexpr int RB = SW * M;
+            constexpr int NQ = RB * CPR / 64;
+            static_assert((RB * CPR) % 64 == 0, "wave block must be whole 16-byte loads");
+            const int64_t wrow = (int64_t)ws * RB;
+            const int64_t col0 = (int64_t)tile * S;
+            u32x4 g[4][NQ];
+            const __amdgpu_buffer_rsrc_t rr[4] = {ra, rb, rc, rd_rsrc};
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int cidx = q * 64 + lane, r = cidx / CPR, h = cidx % CPR;
+                const int off = (int)(((wrow + r) * rs + col0) * (int64_t)sizeof(T)) + h * 16;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    g[k][q] = __builtin_amdgcn_raw_buffer_load_b128(rr[k], off, 0, cpol<NT>());
+            }
+            (void)VV;
+            T rows[4][M];
+            const T *tb = reinterpret_cast<const T *>(xbuf[wave]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (k > 0) wave_lds_sync();
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) xbuf[wave][q * 64 + lane] = g[k][q];
+                wave_lds_sync();
+#pragma unroll
+                for (int j = 0; j < M; ++j) rows[k][j] = tb[(kw * M + j) * S + sl];
+            }
+            RegLoader<T, CH, M> ld{rows};
+            lane_sweep<T, Sh>(o, ld, row0, n, al_store);
+        } else if constexpr (XP) {
             // wave block: 64 segments x M rows = 64*M contiguous elements per array
             constexpr int NQ = 64 * M * (int)sizeof(T) / 1024;   // 16-byte loads per lane
             constexpr int PL = M * (int)sizeof(T) / 16;          // 16-byte pieces per lane row set
@@ -607,6 +645,39 @@ __global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
             xn = (de[j] - alj * xL) - o.ga[j] * xn;
             de[j] = xn;
         }
+    }
+    if constexpr (XP && IL) {
+        // every lane takes part: chunk stores are assigned by position, not by system
+        constexpr int VV = 16 / (int)sizeof(T);
+        constexpr int CPR = S * (int)sizeof(T) / 16;
+        constexpr int RB = SW * M;
+        constexpr int NQ = RB * CPR / 64;
+        const int64_t wrow = (int64_t)ws * RB;
+        const int64_t col0 = (int64_t)tile * S;
+        T *tb = reinterpret_cast<T *>(xbuf[wave]);
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            tb[(kw * M + j) * S + sl] =
+                (j == 0) ? xL : ((j == M - 1) ? xR : de[(j >= 1 && j <= M - 2) ? j - 1 : 0]);
+        wave_lds_sync();
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int cidx = q * 64 + lane, r = cidx / CPR, h = cidx % CPR;
+            const int off = (int)(((wrow + r) * rs + col0) * (int64_t)sizeof(T)) + h * 16;
+            const int64_t s0 = col0 + h * VV;                  // first system of the chunk
+            const u32x4 t = xbuf[wave][cidx];
+            if (s0 + VV <= prm.batch) {
+                __builtin_amdgcn_raw_buffer_store_b128(t, rd_rsrc, off, 0, cpol<NT>());
+            } else {
+#pragma unroll
+                for (int v = 0; v < VV; ++v)
+                    if (s0 + v < prm.batch)
+                        bstore<T, NT>(reinterpret_cast<const T *>(&t)[v], rd_rsrc,
+                                      off + v * (int)sizeof(T), 0);
+            }
+        }
+        return;
     }
     if (!valid) return;
     if constexpr (XP) {
