@@ -5,13 +5,17 @@ one whose answer matches the FP64 reference solution.
     python -m tridiagonalsolver_amd.autotune --prec 0 1 --layout 2 --n 4096 --batch 4096 \
         --out results.json [--table tuning/tuning_gfx950.txt]
 
+    python -m tridiagonalsolver_amd.autotune --ragged --prec 0 1 --table tuning/tuning_gfx950.txt
+
 The table file holds one line per bucket,
     precision layout family n_max M NW S vec lds_alpha nt xp
 (family 0 = the default on-chip solver; nt = nontemporal loads/stores;
 xp = whole-line loads handed to lanes through LDS)
 and is read by libtridiag_amd.so at first use (TRIDIAG_TUNING_FILE overrides its
 location): for a solve of n rows the first bucket with n <= n_max whose kernel
-exists is used.
+exists is used.  --ragged writes lines "R precision n_max shape iface": the shape
+of the ragged persistent kernel and the interface solver for systems of up to
+n_max rows (above the previous line's n_max).
 """
 import argparse
 import ctypes
@@ -112,6 +116,91 @@ def sweep(prec, layout, n, batch, reps=20, warmup=3, nsets=3, seed=1):
     return out
 
 
+def _ragged_inputs(torch, prec, sizes, seed):
+    """Concatenated seeded systems of the given sizes (FP64 master copy)."""
+    off = torch.zeros(len(sizes) + 1, dtype=torch.int64)
+    off[1:] = torch.cumsum(torch.as_tensor(sizes, dtype=torch.int64), 0)
+    rows = int(off[-1])
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    a = torch.rand(rows, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    c = torch.rand(rows, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    b = a.abs() + c.abs() + 1 + torch.rand(rows, generator=g, device="cuda", dtype=torch.float64)
+    d = torch.rand(rows, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    return off, [a, b, c, d]
+
+
+def ragged_sweep(prec, kmin=5, kmax=13, rows_per_class=1 << 23, reps=10, seed=3):
+    """Per size bucket (n in (2^(k-1), 2^k]): time every shape of the ragged kernel
+    that holds 2^k rows with every interface solver, on systems drawn log-uniformly
+    inside the bucket; keep the fastest whose answer matches the FP64 ragged solve."""
+    import torch
+    from .solver import RaggedPlan
+    dtype = torch.float32 if prec == 0 else torch.float64
+    tol = 1e-5 if prec == 0 else 1e-12
+    shapes = _capi.ragged_shapes()
+    out = []
+    for k in range(kmin, kmax + 1):
+        lo, hi = (1 << (k - 1)) + 1 if k > 0 else 1, 1 << k
+        gen = torch.Generator().manual_seed(seed + k)
+        nsys = max(64, rows_per_class // int(0.7 * hi))
+        u = torch.rand(nsys, generator=gen, dtype=torch.float64)
+        sizes = torch.exp(np.log(lo) + (np.log(hi) - np.log(lo)) * u).round().clamp(lo, hi)
+        sizes = sizes.to(torch.int64).tolist()
+        off, m64 = _ragged_inputs(torch, prec, sizes, seed + 100 * k)
+        ref = m64[3].clone()
+        RaggedPlan(off, dtype=torch.float64).solve(m64[0], m64[1], m64[2], ref)
+        work = [t.to(dtype) for t in m64]
+        d0 = work[3].clone()
+        res = []
+        for sh in shapes:
+            if sh["cap"] < hi:
+                continue
+            for iface in range(5):
+                try:
+                    plan = RaggedPlan(off, dtype=dtype, shape=sh["index"], iface=iface)
+                except _capi.TridiagError as e:
+                    _log("skip", sh, iface, e)
+                    continue
+                work[3].copy_(d0)
+                plan.solve(*work)
+                torch.cuda.synchronize()
+                err = float(((work[3].to(torch.float64) - ref).abs().max() / ref.abs().max()).item())
+                stream = torch.cuda.current_stream()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(reps):
+                    plan.solve(*work)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                rows = int(off[-1])
+                w = 4 if prec == 0 else 8
+                res.append(dict(shape=sh["index"], M=sh["M"], NWS=sh["NWS"], S=sh["S"], iface=iface,
+                                ms=ms, gbs=5 * w * rows / (ms * 1e-3) / 1e9, relerr=err))
+                _log("ragged prec %d bucket <=%d: shape %d (M=%d NWS=%d S=%d) iface %d  %.4f ms  "
+                     "%.0f GB/s  err %.2e" % (prec, hi, sh["index"], sh["M"], sh["NWS"], sh["S"],
+                                              iface, ms, res[-1]["gbs"], err))
+        ok = [r for r in res if r["relerr"] <= tol]
+        bestr = min(ok, key=lambda r: r["ms"]) if ok else None
+        out.append(dict(prec=prec, nmax=hi, systems=nsys, results=res, best=bestr))
+        if bestr:
+            _log("best ragged prec %d bucket <=%d -> shape %d iface %d" % (prec, hi, bestr["shape"],
+                                                                          bestr["iface"]))
+    return out
+
+
+def merge_ragged_table(path, buckets):
+    """Replaces the ragged lines ("R precision n_max shape iface") of a table file,
+    keeping its other lines."""
+    keep = []
+    if os.path.exists(path):
+        keep = [ln.rstrip("\n") for ln in open(path) if not ln.startswith("R ")]
+    lines = keep + ["R %d %d %d %d" % (b["prec"], b["nmax"], b["best"]["shape"], b["best"]["iface"])
+                    for b in buckets if b["best"]]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def best(results, tol):
     ok = [r for r in results if r["relerr"] <= tol]
     return min(ok, key=lambda r: r["ms"]) if ok else None
@@ -140,7 +229,21 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--out", default=None)
     ap.add_argument("--table", default=None, help="write the winners as a tuning table")
+    ap.add_argument("--ragged", action="store_true",
+                    help="tune the ragged kernel per size bucket instead (R lines of the table)")
     args = ap.parse_args(argv)
+    if args.ragged:
+        t0 = time.time()
+        buckets = []
+        for prec in args.prec:
+            buckets += ragged_sweep(prec, reps=args.reps)
+        if args.out:
+            with open(args.out, "w") as f:
+                json.dump(buckets, f, indent=1)
+        if args.table:
+            merge_ragged_table(args.table, buckets)
+        _log("done in %.1f s" % (time.time() - t0))
+        return
     allres, rows = [], []
     t0 = time.time()
     for prec in args.prec:
