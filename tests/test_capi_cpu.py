@@ -80,3 +80,32 @@ def test_planner_covers_benchmark_shapes(capi, prec, layout):
 def test_register_packed_cr_rows_per_lane(capi):
     for algo, R in ((11, 4), (12, 8), (13, 16)):
         assert capi.plan(0, algo, 0, 1024, 8)["M"] == R
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_ragged_buckets_hold_their_systems(capi, prec):
+    shapes = {s["index"]: s for s in capi.ragged_shapes()}
+    for n in (1, 2, 31, 32, 33, 100, 128, 129, 1000, 1500, 4096, 4097, 8192):
+        shape, iface = capi.ragged_bucket(prec, n)
+        assert shapes[shape]["cap"] >= n
+        assert 0 <= iface <= 4
+        assert 64 * shapes[shape]["NWS"] * shapes[shape]["S"] == 512
+    with pytest.raises(capi.TridiagError):
+        capi.ragged_bucket(prec, 8193)
+
+
+def test_tuning_file_ragged_lines_are_honoured(tmp_path):
+    """A ragged bucket line of a tuning file overrides the default shape (own process:
+    the table is read once per process)."""
+    import subprocess
+    import sys
+    table = tmp_path / "t.txt"
+    table.write_text("# test table\nR 0 300 2 3\n")   # n <= 300: shape 2 (cap 512), PCR+Thomas
+    code = ("import sys; sys.path.insert(0, %r); import tridiagonalsolver_amd as T; "
+            "print(T.ragged_bucket(0, 200), T.ragged_bucket(0, 301), T.ragged_bucket(1, 200))"
+            % str(ROOT))
+    env = dict(os.environ, TRIDIAG_TUNING_FILE=str(table))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         check=True).stdout.split(") (")
+    assert out[0].strip("( ") == "2, 3"
+    assert out[1] != "2, 3"          # above the line's n_max: default choice
