@@ -109,3 +109,38 @@ def test_tuning_file_ragged_lines_are_honoured(tmp_path):
                          check=True).stdout.split(") (")
     assert out[0].strip("( ") == "2, 3"
     assert out[1] != "2, 3"          # above the line's n_max: default choice
+
+
+def _driver():
+    path = os.path.join(ROOT, "tridiagonalsolver_amd", "lib", "tridiag_test")
+    if not os.path.exists(path):
+        pytest.skip("driver not built")
+    return path
+
+
+def test_driver_lists_reference_algorithms():
+    import subprocess
+    r = subprocess.run([_driver()], capture_output=True, text=True)
+    assert r.returncode == 1                       # usage, like the reference program
+    assert "single-precision algorithms:" in r.stdout
+    assert " 9: reg WM-pGE" in r.stdout and "17: BPLG PCR" in r.stdout
+
+
+@pytest.mark.parametrize("n", [2, 7, 40, 200])
+def test_driver_condition_estimates(n):
+    """Hager-Higham estimates of the driver vs the exact 1- and inf-norm condition
+    numbers of the same generated system (dense numpy inverse)."""
+    import subprocess
+    import numpy as np
+    out = subprocess.run([_driver(), "cond", str(n)], capture_output=True, text=True,
+                         check=True).stdout.split("\n")
+    rows = np.array([[float(v) for v in ln.split()] for ln in out[:n]])
+    c1, ci = (float(v) for v in out[n].split())
+    a, b, c = rows[:, 0], rows[:, 1], rows[:, 2]
+    A = np.diag(b) + np.diag(a[1:], -1) + np.diag(c[:-1], 1)
+    e1, ei = np.linalg.cond(A, 1), np.linalg.cond(A, np.inf)
+    # the estimator is a lower bound, exact or within a small factor on these matrices
+    assert e1 / 3 <= c1 <= e1 * (1 + 1e-12)
+    assert ei / 3 <= ci <= ei * (1 + 1e-12)
+    # diagonally dominant by construction: |b_i| exceeds the off-diagonal row sums
+    assert np.all(np.abs(b) >= np.abs(a) + np.abs(c))

@@ -226,6 +226,15 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, out)
+    # the command-line driver (reference test program interface) against the library
+    drv = os.path.join(LIBDIR, "tridiag_test")
+    r = subprocess.run([HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(os.path.dirname(PKG), "include"),
+                        os.path.join(PKG, "driver", "tridiag_test.cpp"), "-L" + LIBDIR,
+                        "-ltridiag_amd", "-Wl,-rpath,$ORIGIN", "-o", drv + ".tmp"],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n%s\n%s" % (r.stdout, r.stderr))
+    os.replace(drv + ".tmp", drv)
     return out
 
 

@@ -1,0 +1,369 @@
+// Benchmark / check driver with the command line of the reference test program
+// (Tridiagonal_CUDA/test.cpp:141-155):
+//
+//     tridiag_test <precision 0|1> <algorithm> <n> <batch> <rep> [result.csv]
+//
+// It calls the reference-compatible entry points of libtridiag_amd.so through a
+// solver table ordered like Tridiagonal_CUDA/helpers.h:23-66 (init / routine /
+// final hooks, per-solver layout), on one manufactured system replicated over the
+// batch (the recipe of test.cpp:19-38), timed like test.cpp:94-100 (wall clock
+// around each synchronous call, inputs re-initialised untimed before each rep).
+// It prints the same report and appends the same CSV columns (test.cpp:110-133):
+//   size, batch, name, time [s], throughput [unknowns/ns], cond1, condinf,
+//   mean |error|, RMSE, max |error|, argmax
+// The error statistics cover every system of the batch (the reference checks the
+// first), NaN-aware.  The condition numbers use the Hager-Higham 1-norm
+// estimator (Hager 1984; Higham 1988, as in LAPACK xLACON) on FP64 Thomas solves
+// with A and A^T, in place of MKL ?gttrf/?gtcon (test.cpp:41-53).
+// Differences: the random stream is SplitMix64 (the reference draws MKL VSL
+// MT19937, seed 13541532); the distribution recipe is the same.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "tridiagonal.h"
+
+namespace {
+
+template <typename T> struct Solver {
+    void (*routine)(T *, T *, T *, T *, int, int);
+    int (*init)(T *, T *, T *, T *, int, int);
+    void (*final)();
+    int algo;   // id into tridiag_algo_name / tridiag_algo_layout
+};
+
+const Solver<float> kSingle[] = {
+    {cuSparse_single, cuSparse_init_single, cuSparse_final, 0},
+    {Thomas_single, nullptr, nullptr, 1},
+    {CR_single, CR_init_single, CR_final, 2},
+    {PCR5_shmem_single, nullptr, nullptr, 3},
+    {PCR4_shmem_single, nullptr, nullptr, 4},
+    {PCR5_reg_single, nullptr, nullptr, 5},
+    {PCR4_reg_single, nullptr, nullptr, 6},
+    {WM5_shmem_single, nullptr, nullptr, 7},
+    {WM4_shmem_single, nullptr, nullptr, 8},
+    {WM5_reg_single, nullptr, nullptr, 9},
+    {WM4_reg_single, nullptr, nullptr, 10},
+    {CR_reg4_single, nullptr, nullptr, 11},
+    {CR_reg8_single, nullptr, nullptr, 12},
+    {CR_reg16_single, nullptr, nullptr, 13},
+    {BPLG_WM, nullptr, nullptr, 14},
+    {BPLG_LF, nullptr, nullptr, 15},
+    {BPLG_CR, nullptr, nullptr, 16},
+    {BPLG_PCR, nullptr, nullptr, 17},
+};
+const Solver<double> kDouble[] = {
+    {cuSparse_double, cuSparse_init_double, cuSparse_final, 0},
+    {Thomas_double, nullptr, nullptr, 1},
+    {CR_double, CR_init_double, CR_final, 2},
+    {PCR5_shmem_double, nullptr, nullptr, 3},
+    {PCR4_shmem_double, nullptr, nullptr, 4},
+    {PCR5_reg_double, nullptr, nullptr, 5},
+    {PCR4_reg_double, nullptr, nullptr, 6},
+    {WM5_shmem_double, nullptr, nullptr, 7},
+    {WM4_shmem_double, nullptr, nullptr, 8},
+    {WM5_reg_double, nullptr, nullptr, 9},
+    {WM4_reg_double, nullptr, nullptr, 10},
+    {CR_reg4_double, nullptr, nullptr, 11},
+    {CR_reg8_double, nullptr, nullptr, 12},
+    {CR_reg16_double, nullptr, nullptr, 13},
+};
+
+template <typename T> struct Table;
+template <> struct Table<float> {
+    static const Solver<float> *get(int &count) {
+        count = (int)(sizeof(kSingle) / sizeof(kSingle[0]));
+        return kSingle;
+    }
+};
+template <> struct Table<double> {
+    static const Solver<double> *get(int &count) {
+        count = (int)(sizeof(kDouble) / sizeof(kDouble[0]));
+        return kDouble;
+    }
+};
+
+// SplitMix64 (Steele, Lea, Flood 2014): uniform doubles in [lo, hi).
+struct SplitMix64 {
+    uint64_t s;
+    double next(double lo, double hi) {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return lo + (hi - lo) * ((double)(z >> 11) * (1.0 / 9007199254740992.0));
+    }
+};
+
+// One diagonally dominant system with known solution x (recipe of test.cpp:19-38):
+// a_1.., c_..n-2, x ~ U[-1,1], b ~ U[0,1] plus the larger of the two neighbouring
+// off-diagonal row/column sums, d = A x in the working precision.
+template <typename T>
+void generate(std::vector<T> &a, std::vector<T> &b, std::vector<T> &c, std::vector<T> &d,
+              std::vector<T> &x, int n) {
+    SplitMix64 g{13541532ull};
+    a.assign(n, 0);
+    b.assign(n, 0);
+    c.assign(n, 0);
+    d.assign(n, 0);
+    x.assign(n, 0);
+    for (int i = 1; i < n; ++i) a[i] = (T)g.next(-1, 1);
+    for (int i = 0; i < n - 1; ++i) c[i] = (T)g.next(-1, 1);
+    for (int i = 0; i < n; ++i) x[i] = (T)g.next(-1, 1);
+    for (int i = 0; i < n; ++i) b[i] = (T)g.next(0, 1);
+    if (n == 1) {
+        b[0] += 1;
+        d[0] = b[0] * x[0];
+        return;
+    }
+    b[0] += std::max(std::fabs(c[0]), std::fabs(a[1]));
+    d[0] = b[0] * x[0] + c[0] * x[1];
+    for (int i = 1; i < n - 1; ++i) {
+        b[i] += std::max(std::fabs(a[i]) + std::fabs(c[i]), std::fabs(a[i + 1]) + std::fabs(c[i - 1]));
+        d[i] = a[i] * x[i - 1] + b[i] * x[i] + c[i] * x[i + 1];
+    }
+    b[n - 1] += std::max(std::fabs(a[n - 1]), std::fabs(c[n - 2]));
+    d[n - 1] = a[n - 1] * x[n - 2] + b[n - 1] * x[n - 1];
+}
+
+// FP64 Thomas with A (trans = false) or A^T (trans = true) on the host.
+void thomas64(const std::vector<double> &a, const std::vector<double> &b,
+              const std::vector<double> &c, std::vector<double> &x, bool trans) {
+    const int n = (int)b.size();
+    std::vector<double> cp(n), dp(n);
+    auto sub = [&](int i) { return trans ? (i > 0 ? c[i - 1] : 0.0) : a[i]; };
+    auto sup = [&](int i) { return trans ? (i + 1 < n ? a[i + 1] : 0.0) : c[i]; };
+    double cprev = 0, dprev = 0;
+    for (int i = 0; i < n; ++i) {
+        const double l = (i > 0) ? sub(i) : 0.0;
+        const double den = b[i] - l * cprev;
+        cp[i] = (i + 1 < n ? sup(i) : 0.0) / den;
+        dp[i] = (x[i] - l * dprev) / den;
+        cprev = cp[i];
+        dprev = dp[i];
+    }
+    for (int i = n - 1; i >= 0; --i) x[i] = dp[i] - (i + 1 < n ? cp[i] * x[i + 1] : 0.0);
+}
+
+// Hager-Higham estimate of ||A^{-1}||_1 (trans = false) or ||A^{-T}||_1 = ||A^{-1}||_inf.
+double inv_norm1_estimate(const std::vector<double> &a, const std::vector<double> &b,
+                          const std::vector<double> &c, bool trans) {
+    const int n = (int)b.size();
+    std::vector<double> x(n, 1.0 / n), y, z;
+    auto norm1 = [](const std::vector<double> &v) {
+        double s = 0;
+        for (double e : v) s += std::fabs(e);
+        return s;
+    };
+    y = x;
+    thomas64(a, b, c, y, trans);
+    double est = norm1(y);
+    int jprev = -1;
+    for (int iter = 0; iter < 5; ++iter) {
+        z.assign(n, 0);
+        for (int i = 0; i < n; ++i) z[i] = (y[i] >= 0) ? 1.0 : -1.0;
+        thomas64(a, b, c, z, !trans);
+        int j = 0;
+        for (int i = 1; i < n; ++i)
+            if (std::fabs(z[i]) > std::fabs(z[j])) j = i;
+        if (j == jprev) break;
+        jprev = j;
+        y.assign(n, 0);
+        y[j] = 1;
+        thomas64(a, b, c, y, trans);
+        const double e = norm1(y);
+        if (e <= est) break;
+        est = e;
+    }
+    // alternative estimate of Higham 1988 guards against the known bad cases
+    y.assign(n, 0);
+    for (int i = 0; i < n; ++i)
+        y[i] = ((i & 1) ? -1.0 : 1.0) * (1.0 + (n > 1 ? (double)i / (n - 1) : 0.0));
+    thomas64(a, b, c, y, trans);
+    return std::max(est, 2.0 * norm1(y) / (3.0 * n));
+}
+
+template <typename T>
+void condition(const std::vector<T> &a, const std::vector<T> &b, const std::vector<T> &c,
+               double &cond1, double &condinf) {
+    const int n = (int)b.size();
+    std::vector<double> A(a.begin(), a.end()), B(b.begin(), b.end()), C(c.begin(), c.end());
+    A[0] = 0;
+    C[n - 1] = 0;
+    double norm1 = 0, norminf = 0;
+    for (int i = 0; i < n; ++i) {
+        norminf = std::max(norminf, std::fabs(A[i]) + std::fabs(B[i]) + std::fabs(C[i]));
+        const double col = std::fabs(B[i]) + (i + 1 < n ? std::fabs(A[i + 1]) : 0.0) +
+                           (i > 0 ? std::fabs(C[i - 1]) : 0.0);
+        norm1 = std::max(norm1, col);
+    }
+    cond1 = norm1 * inv_norm1_estimate(A, B, C, false);
+    condinf = norminf * inv_norm1_estimate(A, B, C, true);
+}
+
+// Replicates the n-element block at dst[0..n) over batch blocks by doubling copies.
+template <typename T> void replicate(T *dst, int64_t n, int64_t batch) {
+    int64_t have = 1;
+    while (have < batch) {
+        const int64_t k = std::min(have, batch - have);
+        (void)hipMemcpy(dst + have * n, dst, (size_t)(k * n) * sizeof(T), hipMemcpyDeviceToDevice);
+        have += k;
+    }
+}
+
+template <typename T> int run(int algo, int n, int batch, int rep, const char *csv) {
+    int count = 0;
+    const Solver<T> *tab = Table<T>::get(count);
+    const int prec = sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE;
+    if (algo < 0 || algo >= count) return 2;
+    const Solver<T> &sv = tab[algo];
+    const char *name = tridiag_algo_name(prec, sv.algo);
+    const int layout = tridiag_algo_layout(prec, sv.algo);
+    std::vector<T> a, b, c, d, x;
+    generate(a, b, c, d, x, n);
+    const int64_t N = (int64_t)n * batch;
+    T *arr = nullptr;
+    if (hipMalloc((void **)&arr, (size_t)(4 * N) * sizeof(T)) != hipSuccess) {
+        std::cerr << "Not enough GPU memory" << std::endl;
+        return 1;
+    }
+    T *ab = arr, *bb = arr + N, *cb = arr + 2 * N, *db = arr + 3 * N;
+    auto init_batch = [&]() {
+        if (layout == TRIDIAG_LAYOUT_PACKED) {
+            (void)hipMemcpy(arr, a.data(), n * sizeof(T), hipMemcpyHostToDevice);
+            (void)hipMemcpy(arr + n, b.data(), n * sizeof(T), hipMemcpyHostToDevice);
+            (void)hipMemcpy(arr + 2 * n, c.data(), n * sizeof(T), hipMemcpyHostToDevice);
+            (void)hipMemcpy(arr + 3 * n, d.data(), n * sizeof(T), hipMemcpyHostToDevice);
+            replicate(arr, 4 * (int64_t)n, batch);
+        } else {
+            const T *src[4] = {a.data(), b.data(), c.data(), d.data()};
+            T *dst[4] = {ab, bb, cb, db};
+            for (int k = 0; k < 4; ++k) {
+                (void)hipMemcpy(dst[k], src[k], n * sizeof(T), hipMemcpyHostToDevice);
+                replicate(dst[k], n, batch);
+            }
+        }
+        (void)hipDeviceSynchronize();
+    };
+    if (sv.init && sv.init(ab, bb, cb, db, n, batch) != 0) {
+        std::cerr << "Initialization failed" << std::endl;
+        return 1;
+    }
+    std::cout << name << std::endl;
+    double t_total = 0;
+    for (int r = 0; r < rep; ++r) {
+        init_batch();
+        const auto t0 = std::chrono::steady_clock::now();
+        sv.routine(ab, bb, cb, db, n, batch);
+        t_total += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    // x of every system, NaN-aware statistics against the manufactured solution
+    std::vector<T> out((size_t)N);
+    if (layout == TRIDIAG_LAYOUT_PACKED) {
+        std::vector<T> all((size_t)(4 * N));
+        (void)hipMemcpy(all.data(), arr, (size_t)(4 * N) * sizeof(T), hipMemcpyDeviceToHost);
+        for (int64_t s = 0; s < batch; ++s)
+            std::memcpy(&out[(size_t)(s * n)], &all[(size_t)(s * 4 * n + 3 * n)], n * sizeof(T));
+    } else {
+        (void)hipMemcpy(out.data(), db, (size_t)N * sizeof(T), hipMemcpyDeviceToHost);
+    }
+    double e1 = 0, e2 = 0, einf = 0;
+    int64_t maxid = 0;
+    bool nan = false;
+    for (int64_t i = 0; i < N; ++i) {
+        const double err = (double)out[(size_t)i] - (double)x[(size_t)(i % n)];
+        if (std::isnan(err)) {
+            if (!nan) { nan = true; maxid = i; einf = NAN; }
+            continue;
+        }
+        e1 += std::fabs(err);
+        e2 += err * err;
+        if (!nan && std::fabs(err) > einf) { einf = std::fabs(err); maxid = i; }
+    }
+    e1 /= (double)N;
+    e2 = std::sqrt(e2 / (double)N);
+    double cond1 = 0, condinf = 0;
+    condition(a, b, c, cond1, condinf);
+    (void)hipFree(arr);
+    if (sv.final) sv.final();
+    const double t = t_total / rep;
+    const double throughput = 1e-9 * (double)n * batch * rep / t_total;
+    std::cout << std::scientific << "  1-norm condition number: " << cond1
+              << ", inf-norm condition number: " << condinf << std::endl
+              << "Checking error:" << std::endl
+              << "Average: " << e1 << std::endl
+              << "RMSE:    " << e2 << std::endl
+              << "Max:     " << einf << "  (max position: " << maxid << ")" << std::endl
+              << std::fixed << "Size:" << std::setw(7) << n << ",   Time: " << std::setprecision(3)
+              << (t * 1e3) << " ms,   "
+              << "Throughput: " << std::setprecision(6) << throughput << " N/ns" << std::endl
+              << std::endl;
+    if (csv) {
+        std::ofstream(csv, std::ios_base::app)
+            << std::setw(6) << n << ',' << std::setw(8) << batch << ',' << std::setw(25) << name
+            << ',' << std::fixed << std::setw(15) << std::setprecision(6) << t << ','
+            << std::setw(15) << std::setprecision(6) << throughput << ',' << std::scientific
+            << std::setw(15) << std::setprecision(6) << cond1 << ',' << std::setw(15)
+            << std::setprecision(6) << condinf << ',' << std::setw(15) << std::setprecision(6) << e1
+            << ',' << std::setw(15) << std::setprecision(6) << e2 << ',' << std::setw(15)
+            << std::setprecision(6) << einf << ',' << std::setw(6) << maxid << std::endl;
+    }
+    return 0;
+}
+
+void print_algorithms() {
+    for (int prec = 0; prec < 2; ++prec) {
+        std::cout << (prec == 0 ? "single" : "double") << "-precision algorithms:" << std::endl;
+        for (int i = 0; i < tridiag_algo_count(prec); ++i)
+            std::cout << std::setw(3) << i << ": " << tridiag_algo_name(prec, i) << std::endl;
+    }
+}
+
+void print_help(const char *argv0) {
+    std::cout << "usage: " << argv0 << " <precision 0|1> <algorithm> <n> <batch> <rep> [result.csv]"
+              << std::endl;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    std::ios_base::sync_with_stdio(false);
+    if (argc == 3 && std::string(argv[1]) == "cond") {
+        // host-only check mode: the generated FP64 system and its condition estimates
+        const int n = atoi(argv[2]);
+        if (n < 1) return 1;
+        std::vector<double> a, b, c, d, x;
+        generate(a, b, c, d, x, n);
+        double c1 = 0, ci = 0;
+        condition(a, b, c, c1, ci);
+        std::cout << std::setprecision(17);
+        for (int i = 0; i < n; ++i) std::cout << a[i] << ' ' << b[i] << ' ' << c[i] << '\n';
+        std::cout << c1 << ' ' << ci << std::endl;
+        return 0;
+    }
+    if (argc < 6) {
+        print_help(argv[0]);
+        print_algorithms();
+        return 1;
+    }
+    const int prec = atoi(argv[1]), algo = atoi(argv[2]), n = atoi(argv[3]),
+              batch = atoi(argv[4]), rep = atoi(argv[5]);
+    if (prec < 0 || prec > 1 || algo < 0 || n <= 0 || batch <= 0 || rep <= 0 ||
+        algo >= tridiag_algo_count(prec)) {
+        print_help(argv[0]);
+        print_algorithms();
+        return 1;
+    }
+    const char *csv = (argc > 6) ? argv[6] : nullptr;
+    return prec == 0 ? run<float>(algo, n, batch, rep, csv) : run<double>(algo, n, batch, rep, csv);
+}
