@@ -45,6 +45,8 @@ def main(argv=None):
     ap.add_argument("--nsys", type=int, default=65536)
     ap.add_argument("--nmin", type=int, default=32)
     ap.add_argument("--nmax", type=int, default=8192)
+    ap.add_argument("--align", type=int, default=1,
+                    help="ragged: round sizes up to a multiple (1 = as drawn)")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=256)
     args = ap.parse_args(argv)
@@ -65,6 +67,8 @@ def main(argv=None):
         u = torch.rand(args.nsys, generator=gen, dtype=torch.float64)
         lo, hi = torch.log(torch.tensor(float(args.nmin))), torch.log(torch.tensor(float(args.nmax)))
         sizes = torch.exp(lo + (hi - lo) * u).round().clamp(args.nmin, args.nmax).to(torch.int64)
+        if args.align > 1:
+            sizes = ((sizes + args.align - 1) // args.align * args.align).clamp(max=args.nmax)
         off = torch.zeros(args.nsys + 1, dtype=torch.int64)
         off[1:] = torch.cumsum(sizes, 0)
         rows = int(off[-1])
