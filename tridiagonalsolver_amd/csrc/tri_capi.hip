@@ -323,6 +323,18 @@ static int large_impl(int algo, int64_t n, int64_t batch, int64_t sstride, T *a,
     return rc;
 }
 
+// Largest n for which the automatic choice takes the one-system-per-lane Thomas
+// kernel instead of an on-chip tile ($TRIDIAG_LANE_ROWS overrides).
+template <typename T> static int64_t lane_auto_rows(int layout) {
+    static const int64_t env = [] {
+        const char *e = getenv("TRIDIAG_LANE_ROWS");
+        return e ? (int64_t)atoll(e) : (int64_t)-1;
+    }();
+    if (env >= 0) return env;
+    (void)layout;
+    return lane_thomas_max_rows(sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE);
+}
+
 template <typename T>
 static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b, T *c, T *d,
                       hipStream_t st) {
@@ -359,6 +371,21 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
     }
     if (layout == kPacked) {
         b = a + n; c = a + 2 * n; d = a + 3 * n;
+    }
+    // short systems: one system per lane in registers (Thomas), for the Thomas id and
+    // for the automatic choice up to lane_auto_rows()
+    const int32_t lane_bb = buffer_extent<T>(n, batch, sstride, layout);
+    const int prec_id = sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE;
+    if (lane_bb != 0 && n <= lane_thomas_max_rows(prec_id) &&
+        (layout == kInterleaved || layout == kContiguous) &&
+        (algo == kThomas || (algo == TRIDIAG_ALGO_AUTO && n <= lane_auto_rows<T>(layout)))) {
+        hipError_t e = (sizeof(T) == 4)
+            ? launch_lane_thomas_f32((const float *)a, (const float *)b, (const float *)c, (float *)d,
+                                     n, batch, sstride, rstride, layout == kInterleaved, lane_bb, st)
+            : launch_lane_thomas_f64((const double *)a, (const double *)b, (const double *)c,
+                                     (double *)d, n, batch, sstride, rstride, layout == kInterleaved,
+                                     lane_bb, st);
+        return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "lane Thomas launch");
     }
     if (algo == kThomas) {
         hipError_t e = (sizeof(T) == 4)

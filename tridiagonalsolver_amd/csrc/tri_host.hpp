@@ -63,6 +63,14 @@ template <typename T> TileTable<T> tile_table();
 template <> inline TileTable<float> tile_table<float>() { return tile_table_f32(); }
 template <> inline TileTable<double> tile_table<double>() { return tile_table_f64(); }
 
+// Systems of up to lane_thomas_max_rows() rows, one per lane, in registers.
+int lane_thomas_max_rows(int precision);
+hipError_t launch_lane_thomas_f32(const float *a, const float *b, const float *c, float *d,
+                                  int64_t n, int64_t batch, int64_t sstride, int64_t rstride,
+                                  bool il, int32_t bb, hipStream_t st);
+hipError_t launch_lane_thomas_f64(const double *a, const double *b, const double *c, double *d,
+                                  int64_t n, int64_t batch, int64_t sstride, int64_t rstride,
+                                  bool il, int32_t bb, hipStream_t st);
 hipError_t launch_thomas_f32(const float *a, const float *b, float *c, float *d, int64_t n,
                              int64_t batch, int64_t sstride, int64_t rstride, hipStream_t st);
 hipError_t launch_thomas_f64(const double *a, const double *b, double *c, double *d, int64_t n,

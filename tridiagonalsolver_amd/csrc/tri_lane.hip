@@ -1,0 +1,176 @@
+// Short systems, one system per lane, held in registers.
+//
+// The Thomas recurrence of the oracle (Tridiagonal_AVX512/tridiagonal_basic.cpp:6-26):
+//   c'_0 = c_0 / b_0,  d'_0 = d_0 / b_0,
+//   c'_i = c_i / (b_i - a_i c'_{i-1}),  d'_i = (d_i - a_i d'_{i-1}) / (b_i - a_i c'_{i-1}),
+//   x_{n-1} = d'_{n-1},  x_i = d'_i - c'_i x_{i+1},
+// with c' and d' kept in VGPRs (the plain Thomas kernel writes them back to
+// memory), so a solve moves 5 words per unknown.  For systems of up to N rows:
+// the contiguous layout reads each lane's rows as 16-byte pieces, the interleaved
+// layout reads one row of 64 consecutive systems per load instruction.
+#include "tri_host.hpp"
+
+namespace tri {
+
+// Forward and backward Thomas sweeps over rows held in registers (rows past n
+// are identity rows).  x replaces D.
+template <typename T, int N>
+__device__ __forceinline__ void lane_thomas_sweeps(T (&A)[N], T (&B)[N], T (&C)[N], T (&D)[N],
+                                                   int64_t n) {
+    T cp = 0, dp = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const T aj = (j == 0) ? T(0) : A[j];              // a_0 is not part of A
+        const T cj = (j == n - 1) ? T(0) : C[j];          // nor is c_{n-1}
+        const T k = rcp(B[j] - aj * cp);
+        cp = cj * k;
+        dp = (D[j] - aj * dp) * k;
+        C[j] = cp;
+        D[j] = dp;
+    }
+    T xn = 0;
+#pragma unroll
+    for (int j = N - 1; j >= 0; --j) {
+        xn = D[j] - C[j] * xn;
+        D[j] = xn;
+    }
+}
+
+// Interleaved layout: lane = system, one row of 64 consecutive systems per load.
+template <typename T, int N>
+__global__ void __launch_bounds__(256) lane_thomas_il_kernel(const T *a, const T *b, const T *c,
+                                                             T *d, int64_t n, int64_t batch,
+                                                             int64_t rstride, int32_t buf_bytes) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= batch) return;                       // no workgroup barriers below
+    const __amdgpu_buffer_rsrc_t ra = make_rsrc(a, buf_bytes), rb = make_rsrc(b, buf_bytes),
+                                 rc = make_rsrc(c, buf_bytes), rd = make_rsrc(d, buf_bytes);
+    T A[N], B[N], C[N], D[N];
+    const int vo = (int)(s * (int64_t)sizeof(T));
+    const int rowb = (int)(rstride * (int64_t)sizeof(T));
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const bool in = j < n;
+        A[j] = in ? bload<T, true>(ra, vo, j * rowb) : T(0);
+        B[j] = in ? bload<T, true>(rb, vo, j * rowb) : T(1);
+        C[j] = in ? bload<T, true>(rc, vo, j * rowb) : T(0);
+        D[j] = in ? bload<T, true>(rd, vo, j * rowb) : T(0);
+    }
+    lane_thomas_sweeps<T, N>(A, B, C, D, n);
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        if (j < n) bstore<T, true>(D[j], rd, vo, j * rowb);
+}
+
+// Contiguous layout (system stride n): the 64 systems of a wave are 64*n consecutive
+// elements, read as whole 16-byte chunks by consecutive lanes and handed to the lanes
+// through LDS, where system s occupies an odd stride of P >= n elements so that the
+// lanes' row reads hit distinct banks.
+template <typename T, int N>
+__global__ void __launch_bounds__(256) lane_thomas_c_kernel(const T *a, const T *b, const T *c,
+                                                            T *d, int64_t n, int64_t batch) {
+    constexpr int VV = 16 / (int)sizeof(T);
+    constexpr int NQ = (N + VV - 1) / VV;          // chunks per lane at n = N
+    constexpr int PMAX = N | 1;
+    __shared__ T lds[4][64 * PMAX];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t sys0 = ((int64_t)blockIdx.x * 4 + wave) * 64;   // first system of the wave
+    if (sys0 >= batch) return;                                    // wave-uniform, no barriers
+    const int64_t nsw = (batch - sys0 < 64) ? batch - sys0 : 64;   // systems in this wave
+    const int64_t lim = nsw * n;                                    // elements of the wave block
+    const int P = (int)(n | 1);
+    const float inv_n = 1.0f / (float)n;
+    const int32_t nb = (int32_t)(lim * (int64_t)sizeof(T));
+    const int64_t base = sys0 * n;
+    const __amdgpu_buffer_rsrc_t rr[4] = {make_rsrc(a + base, nb), make_rsrc(b + base, nb),
+                                          make_rsrc(c + base, nb), make_rsrc(d + base, nb)};
+    T *L = lds[wave];
+    // element e of the block -> LDS slot (e / n) * P + e % n
+    auto slot = [&](int e) {
+        const int q = (int)(((float)e + 0.5f) * inv_n);
+        return q * P + (e - q * (int)n);
+    };
+    T R[4][N];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        u32x4 g[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int e = (q * 64 + lane) * VV;
+            g[q] = (q * 64 * VV < 64 * n) ? tail_chunk_load<T, true>(rr[k], e, lim)
+                                          : u32x4{0u, 0u, 0u, 0u};
+        }
+        if (k > 0) wave_lds_sync();
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int e = (q * 64 + lane) * VV;
+#pragma unroll
+            for (int v = 0; v < VV; ++v)
+                if (e + v < lim) L[slot(e + v)] = reinterpret_cast<const T *>(&g[q])[v];
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < N; ++j) R[k][j] = (j < n) ? L[lane * P + j] : (k == 1 ? T(1) : T(0));
+    }
+    lane_thomas_sweeps<T, N>(R[0], R[1], R[2], R[3], n);
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        if (j < n) L[lane * P + j] = R[3][j];
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int e = (q * 64 + lane) * VV;
+        if (e >= lim) continue;
+        u32x4 t = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int v = 0; v < VV; ++v)
+            if (e + v < lim) reinterpret_cast<T *>(&t)[v] = L[slot(e + v)];
+        tail_chunk_store<T, true>(t, rr[3], e, lim);
+    }
+}
+
+template <typename T, int N>
+static hipError_t launch_lane(const T *a, const T *b, const T *c, T *d, int64_t n, int64_t batch,
+                              int64_t rstride, bool il, int32_t bb, hipStream_t st) {
+    const int64_t blocks = (batch + 255) / 256;
+    if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+    if (il)
+        hipLaunchKernelGGL((lane_thomas_il_kernel<T, N>), dim3((unsigned)blocks), dim3(256), 0, st,
+                           a, b, c, d, n, batch, rstride, bb);
+    else
+        hipLaunchKernelGGL((lane_thomas_c_kernel<T, N>), dim3((unsigned)blocks), dim3(256), 0, st,
+                           a, b, c, d, n, batch);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_lane_thomas(const T *a, const T *b, const T *c, T *d, int64_t n,
+                                     int64_t batch, int64_t sstride, int64_t rstride, bool il,
+                                     int32_t bb, hipStream_t st) {
+    if (n < 1 || n > lane_thomas_max_rows(sizeof(T) == 4 ? 0 : 1)) return hipErrorInvalidValue;
+    if (!il && sstride != n) return hipErrorInvalidValue;   // contiguous systems only
+    if (batch <= 0) return hipSuccess;
+#define TRI_LANE(NN) \
+    if (n <= NN) return launch_lane<T, NN>(a, b, c, d, n, batch, rstride, il, bb, st);
+    TRI_LANE(8)
+    TRI_LANE(16)
+    if constexpr (sizeof(T) == 4) { TRI_LANE(32) }
+#undef TRI_LANE
+    return hipErrorInvalidValue;
+}
+
+int lane_thomas_max_rows(int precision) { return precision == 0 ? 32 : 16; }
+
+hipError_t launch_lane_thomas_f32(const float *a, const float *b, const float *c, float *d,
+                                  int64_t n, int64_t batch, int64_t sstride, int64_t rstride,
+                                  bool il, int32_t bb, hipStream_t st) {
+    return launch_lane_thomas<float>(a, b, c, d, n, batch, sstride, rstride, il, bb, st);
+}
+hipError_t launch_lane_thomas_f64(const double *a, const double *b, const double *c, double *d,
+                                  int64_t n, int64_t batch, int64_t sstride, int64_t rstride,
+                                  bool il, int32_t bb, hipStream_t st) {
+    return launch_lane_thomas<double>(a, b, c, d, n, batch, sstride, rstride, il, bb, st);
+}
+
+}  // namespace tri
