@@ -207,11 +207,16 @@ def best(results, tol):
 
 
 def write_table(path, rows):
-    """rows: list of (prec, layout, n_max, entry) tuples."""
-    os.makedirs(os.path.dirname(path), exist_ok=True)
+    """rows: list of (prec, layout, n_max, entry) tuples.  Ragged ("R") lines of an
+    existing table are kept."""
+    os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+    ragged = [ln for ln in open(path)] if os.path.exists(path) else []
+    ragged = [ln.rstrip("\n") for ln in ragged if ln.startswith("R ")]
     with open(path, "w") as f:
         f.write("# precision layout family n_max M NW S vec lds_alpha nt xp  "
                 "(written by autotune.py)\n")
+        for ln in ragged:
+            f.write(ln + "\n")
         for prec, layout, nmax, e in sorted(rows, key=lambda r: (r[0], r[1], r[2])):
             f.write("%d %d 0 %d %d %d %d %d %d %d %d\n" % (
                 prec, layout, nmax, e["M"], e["NW"], e["S"], int(e["vec"]), int(e["lds_alpha"]),

@@ -488,6 +488,13 @@ __device__ __forceinline__ void interface_solve(T aL, T DLr, T ELr, T FLr, T ARr
     xR = oR.r - oR.p * u - oR.q * v;
 }
 
+// Occupancy floor of a tile kernel: 512-lane transposed-load tiles are held to two
+// workgroups per CU (4 waves per SIMD, 128 VGPRs); one workgroup of 8 waves leaves
+// too few loads in flight to stream at HBM speed.
+template <typename T, int M, int NW, bool XP> constexpr int tile_min_waves() {
+    return (XP && NW == 8 && M * (int)sizeof(T) <= 64) ? 4 : 1;
+}
+
 // PAD = false: the system fills the tile exactly (n == CAP), loads are not
 // predicated.  PAD = true: n < CAP, rows past the end are identity rows.
 // BIG: 64-bit addressing for arrays of 2 GiB or more (always predicated).
@@ -499,7 +506,8 @@ __device__ __forceinline__ void interface_solve(T aL, T DLr, T ELr, T FLr, T ARr
 // whole cache lines, which is what the nontemporal policy needs.
 template <typename T, int M, int NW, int S, bool IL, bool VEC, bool ALDS, bool BIG, bool PAD,
           bool NT, bool XP>
-__global__ void __launch_bounds__(64 * NW) tile_wm_kernel(TileParams<T> prm) {
+__global__ void __launch_bounds__(64 * NW)
+    __attribute__((amdgpu_waves_per_eu(tile_min_waves<T, M, NW, XP>()))) tile_wm_kernel(TileParams<T> prm) {
     using Sh = TileShape<T, M, NW, S, IL, VEC, ALDS, BIG, PAD, NT, XP>;
     static_assert(!XP || (!PAD && !BIG && (IL ? (S * (int)sizeof(T)) % 16 == 0 : VEC)),
                   "transposed loads need exact tiles (and 16-byte row pieces when interleaved)");
