@@ -185,6 +185,24 @@ int tridiag_solve_fwd_bwd(int precision, int layout, int64_t n_eqt, int64_t n_ba
                           const void *g, void *x, void *ga, void *gb, void *gc, void *gd,
                           void *stream);
 
+/* Layout conversion of the coefficient arrays.  No counterpart symbol in the
+ * reference: its driver builds each solver's layout while it replicates the
+ * batch (init_batch, Tridiagonal_CUDA/utils.cu:263-297); this lets callers that
+ * hold one layout use a solver whose fast path is another.  Element (coefficient k, system s, row i) sits at
+ *   layout 0: ptr_k[s*n_eqt + i]       (four arrays a, b, c, d)
+ *   layout 1: buf[s*4*n_eqt + k*n_eqt + i]  (one buffer, passed as *_a; the
+ *             *_b, *_c, *_d pointers on that side are ignored)
+ *   layout 2: ptr_k[i*n_batch + s]      (four arrays)
+ * Between layouts 0 and 2 a null pointer skips that coefficient (source and
+ * destination must then both be null), e.g. to convert the solution d alone.
+ * With a packed side, the coefficients converted are those given on the other
+ * side.  Source and destination may not overlap.  Asynchronous on `stream`;
+ * bit-exact (a permutation of elements). */
+int tridiag_convert_layout(int precision, int src_layout, int dst_layout, int64_t n_eqt,
+                           int64_t n_batch, const void *src_a, const void *src_b,
+                           const void *src_c, const void *src_d, void *dst_a, void *dst_b,
+                           void *dst_c, void *dst_d, void *stream);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *tridiag_last_error(void);
 

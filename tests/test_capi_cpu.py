@@ -144,3 +144,34 @@ def test_driver_condition_estimates(n):
     assert ei / 3 <= ci <= ei * (1 + 1e-12)
     # diagonally dominant by construction: |b_i| exceeds the off-diagonal row sums
     assert np.all(np.abs(b) >= np.abs(a) + np.abs(c))
+
+
+def test_convert_layout_validation_without_gpu(capi):
+    """Argument checks of tridiag_convert_layout that fail before any device call."""
+    L = capi.lib()
+    nul = None
+    p = [ctypes.c_void_p(0x100000 * (k + 1)) for k in range(8)]   # never dereferenced
+
+    def conv(prec, sl, dl, n, batch, src, dst):
+        return L.tridiag_convert_layout(prec, sl, dl, n, batch, *src, *dst, nul)
+
+    assert conv(2, 0, 2, 8, 8, p[:4], p[4:]) == capi.EINVAL               # precision
+    assert conv(0, 3, 2, 8, 8, p[:4], p[4:]) == capi.EINVAL               # layout
+    assert conv(0, 0, -1, 8, 8, p[:4], p[4:]) == capi.EINVAL
+    assert conv(0, 0, 2, 0, 8, p[:4], p[4:]) == capi.EINVAL               # n_eqt
+    assert conv(0, 0, 2, 8, -1, p[:4], p[4:]) == capi.EINVAL              # n_batch
+    assert conv(0, 0, 2, 1 << 40, 1 << 40, p[:4], p[4:]) == capi.EINVAL   # overflow
+    # a coefficient given on one side only
+    assert conv(0, 0, 2, 8, 8, p[:4], [p[4], p[5], p[6], nul]) == capi.EINVAL
+    assert b"coefficient 3" in L.tridiag_last_error()
+    assert conv(0, 0, 2, 8, 8, [nul] * 4, [nul] * 4) == capi.EINVAL       # nothing to do
+    # packed side without its buffer
+    assert conv(0, 1, 0, 8, 8, [nul] * 4, p[4:]) == capi.EINVAL
+    assert conv(0, 0, 1, 8, 8, p[:4], [nul] * 4) == capi.EINVAL
+    # overlap: same array, and a destination inside the packed source buffer
+    assert conv(0, 0, 2, 8, 8, p[:4], p[:4]) == capi.EINVAL
+    assert b"overlap" in L.tridiag_last_error()
+    inside = ctypes.c_void_p(p[0].value + 4 * 8)
+    assert conv(0, 1, 0, 8, 8, [p[0], nul, nul, nul], [nul, nul, nul, inside]) == capi.EINVAL
+    # an empty batch after valid arguments is a no-op
+    assert conv(1, 0, 2, 8, 0, p[:4], p[4:]) == capi.OK

@@ -8,5 +8,5 @@ calls, plus the interleaved layout, ragged batches and the adjoint.
 from ._capi import (ALGO_AUTO, CONTIGUOUS, DOUBLE, INTERLEAVED, PACKED, SINGLE,  # noqa: F401
                     TridiagError, algo_names, check, lib, plan, ragged_bucket,
                     ragged_shapes, tiles)
-from .solver import (RaggedPlan, solve, solve_adjoint, solve_fwd_bwd,  # noqa: F401
-                     solve_ragged, TridiagonalSolve, tridiagonal_solve)
+from .solver import (RaggedPlan, convert_layout, solve, solve_adjoint,  # noqa: F401
+                     solve_fwd_bwd, solve_ragged, TridiagonalSolve, tridiagonal_solve)

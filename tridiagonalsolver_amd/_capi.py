@@ -32,7 +32,7 @@ EXTENDED_SYMBOLS = [
     "tridiag_solve_fwd_bwd", "tridiag_last_error", "tridiag_plan", "tridiag_tile_count",
     "tridiag_tile_info", "tridiag_solve_tile", "tridiag_ragged_shape_count",
     "tridiag_ragged_shape_info", "tridiag_ragged_bucket", "tridiag_ragged_plan",
-    "tridiag_ragged_solve",
+    "tridiag_ragged_solve", "tridiag_convert_layout",
 ]
 
 
@@ -93,6 +93,8 @@ def lib():
     L.tridiag_ragged_plan.restype = ci
     L.tridiag_ragged_solve.argtypes = [ci, i64, vp, vp, vp, vp, vp, vp]
     L.tridiag_ragged_solve.restype = ci
+    L.tridiag_convert_layout.argtypes = [ci, ci, ci, i64, i64] + [vp] * 9
+    L.tridiag_convert_layout.restype = ci
     _lib = L
     return L
 

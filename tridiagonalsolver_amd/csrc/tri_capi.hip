@@ -713,6 +713,74 @@ static int adjoint_impl(int mode, int algo, int layout, int64_t n, int64_t batch
 using namespace tri;
 
 // ======================================================================= Part 2
+// Layout conversion.  Element (coefficient k, system s, row i) sits at
+// ptr_k[s*n + i] (layout 0), buf[s*4n + k*n + i] (layout 1) or ptr_k[i*batch + s]
+// (layout 2); layouts 0 and 1 are row-pitched views of the same system-major
+// matrix, layout 2 its transpose.
+template <typename T>
+static int convert_impl(int src_layout, int dst_layout, int64_t n, int64_t batch,
+                        const T *const src[4], T *const dst[4], hipStream_t st) {
+    if (src_layout < 0 || src_layout > 2 || dst_layout < 0 || dst_layout > 2)
+        return fail(TRIDIAG_EINVAL, "layouts must be 0, 1 or 2");
+    if (n < 1) return fail(TRIDIAG_EINVAL, "n_eqt must be >= 1");
+    if (batch < 0) return fail(TRIDIAG_EINVAL, "n_batch must be >= 0");
+    if (batch > 0 && n > INT64_MAX / 4 / batch)
+        return fail(TRIDIAG_EINVAL, "n_eqt * n_batch is too large");
+    if (src_layout == kPacked && !src[0])
+        return fail(TRIDIAG_EINVAL, "packed source: the buffer is passed as src_a");
+    if (dst_layout == kPacked && !dst[0])
+        return fail(TRIDIAG_EINVAL, "packed destination: the buffer is passed as dst_a");
+    const int64_t elems = n * batch;
+    LayoutMove<T> p{};
+    int count = 0;
+    for (int k = 0; k < 4; ++k) {
+        const bool has_s = src_layout == kPacked || src[k];
+        const bool has_d = dst_layout == kPacked || dst[k];
+        if (src_layout != kPacked && dst_layout != kPacked && has_s != has_d)
+            return fail(TRIDIAG_EINVAL, "coefficient " + std::to_string(k) +
+                                            ": source and destination must both be given or "
+                                            "both be null");
+        if (!has_s || !has_d) continue;
+        const T *sp = src_layout == kPacked ? src[0] + k * n : src[k];
+        T *dp = dst_layout == kPacked ? dst[0] + k * n : dst[k];
+        // whole extent of each side, for the overlap check
+        const T *s_lo = src_layout == kPacked ? src[0] : sp;
+        const T *d_lo = dst_layout == kPacked ? dst[0] : dp;
+        const int64_t s_len = src_layout == kPacked ? 4 * elems : elems;
+        const int64_t d_len = dst_layout == kPacked ? 4 * elems : elems;
+        const uintptr_t s0 = reinterpret_cast<uintptr_t>(s_lo), d0 = reinterpret_cast<uintptr_t>(d_lo);
+        if (elems && s0 < d0 + (uintptr_t)d_len * sizeof(T) && d0 < s0 + (uintptr_t)s_len * sizeof(T))
+            return fail(TRIDIAG_EINVAL, "coefficient " + std::to_string(k) +
+                                            ": source and destination overlap (in-place "
+                                            "conversion is not supported)");
+        p.src[count] = sp;
+        p.dst[count] = dp;
+        ++count;
+    }
+    if (count == 0) return fail(TRIDIAG_EINVAL, "no coefficient array to convert");
+    if (batch == 0) return TRIDIAG_OK;
+    const bool src_sys = src_layout != kInterleaved, dst_sys = dst_layout != kInterleaved;
+    const int64_t ps = src_layout == kPacked ? 4 * n : n;
+    const int64_t pd = dst_layout == kPacked ? 4 * n : n;
+    bool transpose = false;
+    if (src_sys && dst_sys) {              // system-major to system-major: pitched copy
+        p.rows = batch; p.cols = n; p.ld_src = ps; p.ld_dst = pd;
+        if (ps == n && pd == n) { p.rows = 1; p.cols = elems; p.ld_src = p.ld_dst = elems; }
+    } else if (src_sys) {                  // to interleaved: transpose of batch x n
+        p.rows = batch; p.cols = n; p.ld_src = ps; p.ld_dst = batch;
+        transpose = true;
+    } else if (dst_sys) {                  // from interleaved: transpose of n x batch
+        p.rows = n; p.cols = batch; p.ld_src = batch; p.ld_dst = pd;
+        transpose = true;
+    } else {                               // interleaved to interleaved: flat copy
+        p.rows = 1; p.cols = elems; p.ld_src = p.ld_dst = elems;
+    }
+    hipError_t e;
+    if constexpr (sizeof(T) == 4) e = launch_layout_move_f32(p, count, transpose, st);
+    else e = launch_layout_move_f64(p, count, transpose, st);
+    return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "layout conversion launch");
+}
+
 extern "C" {
 
 int tridiag_algo_count(int precision) {
@@ -901,6 +969,27 @@ int tridiag_solve_fwd_bwd(int precision, int layout, int64_t n_eqt, int64_t n_ba
                                     (const double *)b, (const double *)c, (const double *)d,
                                     nullptr, (const double *)g, (double *)x, (double *)ga,
                                     (double *)gb, (double *)gc, (double *)gd, st);
+    return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
+}
+
+int tridiag_convert_layout(int precision, int src_layout, int dst_layout, int64_t n_eqt,
+                           int64_t n_batch, const void *src_a, const void *src_b,
+                           const void *src_c, const void *src_d, void *dst_a, void *dst_b,
+                           void *dst_c, void *dst_d, void *stream) {
+    g_last_error.clear();
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == TRIDIAG_SINGLE) {
+        const float *s[4] = {(const float *)src_a, (const float *)src_b, (const float *)src_c,
+                             (const float *)src_d};
+        float *d[4] = {(float *)dst_a, (float *)dst_b, (float *)dst_c, (float *)dst_d};
+        return convert_impl<float>(src_layout, dst_layout, n_eqt, n_batch, s, d, st);
+    }
+    if (precision == TRIDIAG_DOUBLE) {
+        const double *s[4] = {(const double *)src_a, (const double *)src_b,
+                              (const double *)src_c, (const double *)src_d};
+        double *d[4] = {(double *)dst_a, (double *)dst_b, (double *)dst_c, (double *)dst_d};
+        return convert_impl<double>(src_layout, dst_layout, n_eqt, n_batch, s, d, st);
+    }
     return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
 }
 

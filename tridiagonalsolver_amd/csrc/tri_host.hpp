@@ -71,6 +71,18 @@ hipError_t launch_lane_thomas_f32(const float *a, const float *b, const float *c
 hipError_t launch_lane_thomas_f64(const double *a, const double *b, const double *c, double *d,
                                   int64_t n, int64_t batch, int64_t sstride, int64_t rstride,
                                   bool il, int32_t bb, hipStream_t st);
+// Layout conversion of up to four coefficient arrays: each source is a rows x cols
+// matrix with row pitch ld_src, copied (transpose = false) or transposed into a
+// destination with row pitch ld_dst.  col_tiles is filled by the launcher.
+template <typename T> struct LayoutMove {
+    const T *src[4];
+    T *dst[4];
+    int64_t rows, cols, ld_src, ld_dst, col_tiles;
+};
+hipError_t launch_layout_move_f32(const LayoutMove<float> &p, int count, bool transpose,
+                                  hipStream_t st);
+hipError_t launch_layout_move_f64(const LayoutMove<double> &p, int count, bool transpose,
+                                  hipStream_t st);
 hipError_t launch_thomas_f32(const float *a, const float *b, float *c, float *d, int64_t n,
                              int64_t batch, int64_t sstride, int64_t rstride, hipStream_t st);
 hipError_t launch_thomas_f64(const double *a, const double *b, double *c, double *d, int64_t n,

@@ -61,6 +61,48 @@ def solve(a, b, c, d, algo=ALGO_AUTO, layout=CONTIGUOUS, n=None, batch=None):
     return a if layout == PACKED else d
 
 
+def convert_layout(arrays, src_layout, dst_layout, n=None, batch=None):
+    """Copies coefficient arrays into another layout (a permutation, bit-exact).
+
+    `arrays` is a packed (batch, 4n) tensor for layout 1, else a sequence of four
+    tensors (a, b, c, d) in which None skips that coefficient.  Returns a new
+    packed tensor for layout 1, else a tuple of four tensors, (batch, n) for
+    layout 0 and (n, batch) for layout 2, with None where skipped.  A packed
+    destination needs all four coefficients.
+    """
+    if src_layout == PACKED:
+        src = [arrays]
+        ref = arrays
+    else:
+        src = list(arrays)
+        if len(src) != 4:
+            raise ValueError("pass four arrays (a, b, c, d); None skips one")
+        ref = next((t for t in src if t is not None), None)
+        if ref is None:
+            raise ValueError("no array to convert")
+    given = [t for t in src if t is not None]
+    _check_tensors(given)
+    if n is None or batch is None:
+        n, batch = _shape(ref, src_layout, n, batch)
+    n, batch = int(n), int(batch)
+    if dst_layout == PACKED:
+        if src_layout != PACKED and len(given) != 4:
+            raise ValueError("a packed destination needs all four coefficients")
+        out = torch.empty((batch, 4 * n), dtype=ref.dtype, device=ref.device)
+        dptr = [_ptr(out), None, None, None]
+    else:
+        shape = (n, batch) if dst_layout == INTERLEAVED else (batch, n)
+        have = [True] * 4 if src_layout == PACKED else [t is not None for t in src]
+        outs = [torch.empty(shape, dtype=ref.dtype, device=ref.device) if h else None
+                for h in have]
+        dptr = [_ptr(o) if o is not None else None for o in outs]
+    sptr = [_ptr(src[0]), None, None, None] if src_layout == PACKED else \
+        [_ptr(t) if t is not None else None for t in src]
+    check(lib().tridiag_convert_layout(_precision(ref), src_layout, dst_layout, n, batch,
+                                       *sptr, *dptr, _stream_handle(ref.device)))
+    return out if dst_layout == PACKED else tuple(outs)
+
+
 class RaggedPlan:
     """Schedule of a ragged batch: system s = rows [offsets[s], offsets[s+1]) of 1-D
     tensors a, b, c, d (contiguous per system, concatenated), planned once on the
