@@ -190,7 +190,7 @@ def test_graph_replay_matches_eager():
 
 
 def test_repeated_solves_reset_queue():
-    """Back-to-back solves on one plan (the work-queue head is reset each time)."""
+    """Back-to-back solves on one plan: each single launch resets its own queue."""
     import torch
     import tridiagonalsolver_amd as T
     sizes = oracle.ragged_sizes(700, 1, 4000, seed=31)
@@ -203,7 +203,11 @@ def test_repeated_solves_reset_queue():
         d = src[3].clone()
         plan.solve(src[0], src[1], src[2], d)
         outs.append(d)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        # the last workgroup of the launch leaves both queue counters at zero
+        # (header bytes 24..27 = queue head, 32..35 = finished workgroups)
+        hdr = plan.work[:64].cpu().numpy()
+        assert hdr[24:28].view(np.int32)[0] == 0 and hdr[32:36].view(np.int32)[0] == 0
     assert all(torch.equal(outs[0], o) for o in outs[1:])
     _check(off, arrs, outs[0].cpu().numpy(), np.float64)
 

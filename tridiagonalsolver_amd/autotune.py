@@ -130,17 +130,21 @@ def _ragged_inputs(torch, prec, sizes, seed):
 
 
 def ragged_sweep(prec, kmin=5, kmax=13, rows_per_class=1 << 23, reps=10, seed=3):
-    """Per size bucket (n in (2^(k-1), 2^k]): time every shape of the ragged kernel
-    that holds 2^k rows with every interface solver, on systems drawn log-uniformly
-    inside the bucket; keep the fastest whose answer matches the FP64 ragged solve."""
+    """Per size bucket: time every shape of the ragged kernel that holds the bucket's
+    largest n with every interface solver, on systems drawn log-uniformly inside the
+    bucket; keep the fastest whose answer matches the FP64 ragged solve.  Bucket
+    bounds are the powers of two 2^kmin..2^kmax and every shape capacity between
+    them, so each bucket ends at a capacity."""
     import torch
     from .solver import RaggedPlan
     dtype = torch.float32 if prec == 0 else torch.float64
     tol = 1e-5 if prec == 0 else 1e-12
     shapes = _capi.ragged_shapes()
+    bounds = sorted({1 << k for k in range(kmin, kmax + 1)} |
+                    {sh["cap"] for sh in shapes if (1 << kmin) <= sh["cap"] <= (1 << kmax)})
     out = []
-    for k in range(kmin, kmax + 1):
-        lo, hi = (1 << (k - 1)) + 1 if k > 0 else 1, 1 << k
+    for k, hi in enumerate(bounds):
+        lo = bounds[k - 1] + 1 if k > 0 else (hi >> 1) + 1
         gen = torch.Generator().manual_seed(seed + k)
         nsys = max(64, rows_per_class // int(0.7 * hi))
         u = torch.rand(nsys, generator=gen, dtype=torch.float64)

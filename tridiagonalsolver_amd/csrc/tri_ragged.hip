@@ -3,12 +3,6 @@
 
 namespace tri {
 
-// Zeroes the work-queue head.  A kernel rather than a memset so that a solve
-// captured into a graph replays it as an ordinary kernel node.
-__global__ void ragged_reset_kernel(RaggedHeader *h) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) h->counter = 0;
-}
-
 // Persistent grid: as many workgroups as fit on the device at once.
 template <typename T> static int ragged_grid() {
     static int g = [] {
@@ -35,9 +29,6 @@ static hipError_t launch_ragged(const RaggedParams<T> &p, int grid, hipStream_t 
                                    ragged_lds_bytes<T>());
     }();
     if (granted != hipSuccess) return granted;
-    hipLaunchKernelGGL(ragged_reset_kernel, dim3(1), dim3(64), 0, st, p.hdr);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(ragged_kernel<T>, dim3(grid), dim3(kRaggedLanes), ragged_lds_bytes<T>(), st,
                        p);
     return hipGetLastError();

@@ -21,10 +21,13 @@ namespace tri {
 struct RaggedShape {
     int M, NWS, S;
 };
-constexpr int kRaggedShapes = 12;
+constexpr int kRaggedShapes = 16;
 constexpr RaggedShape kRaggedShapeTab[kRaggedShapes] = {
     {2, 1, 8},  {4, 1, 8},  {8, 1, 8},  {8, 2, 4},  {16, 1, 8}, {8, 4, 2},
-    {16, 2, 4}, {8, 8, 1},  {16, 4, 2}, {16, 8, 1}, {4, 2, 4},  {4, 4, 2}};
+    {16, 2, 4}, {8, 8, 1},  {16, 4, 2}, {16, 8, 1}, {4, 2, 4},  {4, 4, 2},
+    // 12 rows per lane: capacities 1.5x a power of two, so that systems between
+    // two powers of two leave fewer padding rows
+    {12, 1, 8}, {12, 2, 4}, {12, 4, 2}, {12, 8, 1}};
 constexpr int ragged_cap(int k) { return 64 * kRaggedShapeTab[k].NWS * kRaggedShapeTab[k].M; }
 constexpr int kRaggedLanes = 512;
 constexpr int kRaggedWaves = kRaggedLanes / 64;
@@ -35,16 +38,19 @@ constexpr int kRaggedMaxM = 16;
 //   r[0] shape, r[1] interface solver, r[2] number of systems, r[3] 0,
 //   r[4 + 3k], r[5 + 3k]  first row of system slot k (int64, low word first),
 //   r[6 + 3k]             rows of system slot k.
-// Written by the host planner once; a solve only resets `counter`.
+// Written by the host planner once (with both counters zero).  A solve leaves the
+// header as it found it: the last workgroup to finish zeroes both counters.
 struct RaggedHeader {
     int64_t magic;
     int64_t n_sys;
     int64_t nitems;
-    int32_t counter;     // work-queue head, zeroed before every solve
+    int32_t counter;     // work-queue head
     int32_t precision;
+    int32_t done;        // workgroups of the running solve that have left the queue
 };
 constexpr int64_t kRaggedMagic = 0x52414747454432LL;
 constexpr int64_t kRaggedHeaderBytes = 64;
+static_assert(sizeof(RaggedHeader) <= kRaggedHeaderBytes, "header exceeds its 64 bytes");
 constexpr int kRaggedRecInts = 32;
 constexpr int kRaggedMaxSlots = 8;
 
@@ -71,6 +77,7 @@ __device__ __forceinline__ void ragged_item(const RaggedParams<T> &p, const int3
     constexpr int NCH = XPL ? 64 * M / VV : 1;               // 16-byte chunks per wave block
     constexpr int NQ = XPL ? NCH / 64 : 1;                   // chunks per lane
     constexpr int PL = XPL ? M / VV : 1;                     // chunks per lane segment
+    static_assert(!XPL || M % VV == 0, "whole-chunk loads need whole chunks per lane");
     using Sh = TileShape<T, M, S * NWS, S, false, false, false, false, true, true, XPL>;
     constexpr int SW = 64, CH = Sh::CH;
     // Opaque copy of the lane id: without it the lane-dependent values of every
@@ -232,6 +239,7 @@ __global__ void __launch_bounds__(kRaggedLanes) ragged_kernel(RaggedParams<T> p)
             TRI_RAGGED_CASE(0) TRI_RAGGED_CASE(1) TRI_RAGGED_CASE(2) TRI_RAGGED_CASE(3)
             TRI_RAGGED_CASE(4) TRI_RAGGED_CASE(5) TRI_RAGGED_CASE(6) TRI_RAGGED_CASE(7)
             TRI_RAGGED_CASE(8) TRI_RAGGED_CASE(9) TRI_RAGGED_CASE(10) TRI_RAGGED_CASE(11)
+            TRI_RAGGED_CASE(12) TRI_RAGGED_CASE(13) TRI_RAGGED_CASE(14) TRI_RAGGED_CASE(15)
 #undef TRI_RAGGED_CASE
         default: break;
         }
@@ -242,6 +250,20 @@ __global__ void __launch_bounds__(kRaggedLanes) ragged_kernel(RaggedParams<T> p)
             nxt = (nxt < nitems) ? nn : nxt;
         }
         __syncthreads();                     // next record visible; LDS of the item free
+    }
+    // Thread 0 made every queue increment of this workgroup, each completed (its
+    // value was used) before this point.  The workgroup that arrives last therefore
+    // sees every other workgroup's increments done and resets the queue for the next
+    // solve, which is thus a single launch (no reset kernel or memset to replay).
+    // Relaxed order suffices (the returned values order the atomics); an acq_rel
+    // increment would write back and invalidate the L2 of every workgroup's XCD.
+    if (tid == 0) {
+        const int32_t before = __hip_atomic_fetch_add(&p.hdr->done, 1, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+        if (before == (int32_t)gridDim.x - 1) {
+            __hip_atomic_store(&p.hdr->counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&p.hdr->done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
