@@ -388,6 +388,16 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
         return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "lane Thomas launch");
     }
     if (algo == kThomas) {
+        if (layout != kInterleaved && lane_bb != 0) {
+            // rows of each lane's system as 16-byte chunks (aligned rows below 2 GiB)
+            hipError_t e = (sizeof(T) == 4)
+                ? launch_thomas_rows_f32((const float *)a, (const float *)b, (float *)c, (float *)d,
+                                         n, batch, sstride, lane_bb, st)
+                : launch_thomas_rows_f64((const double *)a, (const double *)b, (double *)c,
+                                         (double *)d, n, batch, sstride, lane_bb, st);
+            if (e != hipErrorInvalidValue)
+                return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "Thomas launch");
+        }
         hipError_t e = (sizeof(T) == 4)
             ? launch_thomas_f32((const float *)a, (const float *)b, (float *)c, (float *)d, n,
                                 batch, sstride, rstride, st)

@@ -83,6 +83,13 @@ hipError_t launch_layout_move_f32(const LayoutMove<float> &p, int count, bool tr
                                   hipStream_t st);
 hipError_t launch_layout_move_f64(const LayoutMove<double> &p, int count, bool transpose,
                                   hipStream_t st);
+// Thomas in the contiguous / packed layouts with 16-byte chunked rows per lane;
+// hipErrorInvalidValue when rows are not 16-byte aligned or an array spans 2 GiB.
+hipError_t launch_thomas_rows_f32(const float *a, const float *b, float *c, float *d, int64_t n,
+                                  int64_t batch, int64_t sstride, int32_t bb, hipStream_t st);
+hipError_t launch_thomas_rows_f64(const double *a, const double *b, double *c, double *d,
+                                  int64_t n, int64_t batch, int64_t sstride, int32_t bb,
+                                  hipStream_t st);
 hipError_t launch_thomas_f32(const float *a, const float *b, float *c, float *d, int64_t n,
                              int64_t batch, int64_t sstride, int64_t rstride, hipStream_t st);
 hipError_t launch_thomas_f64(const double *a, const double *b, double *c, double *d, int64_t n,

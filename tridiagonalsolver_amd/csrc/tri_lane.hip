@@ -160,6 +160,165 @@ static hipError_t launch_lane_thomas(const T *a, const T *b, const T *c, T *d, i
     return hipErrorInvalidValue;
 }
 
+// ---------------------------------------------------------------- longer systems
+// Thomas over systems of any length in the contiguous (and packed) layout, one
+// system per lane, with the reference's memory semantics: c' is written to c and
+// d' to d on the way down, x to d on the way up (9 words per unknown).  Each lane
+// walks its own rows in chunks of R rows read and written as 16-byte vectors, and
+// loads the next chunk while it eliminates the current one, so a system costs
+// about 2n/R dependent memory round trips instead of 2n.
+
+// Chunk of 16 bytes at row e of the system starting at element `base` of the
+// array behind r; rows at or past n read as zero and are never touched.
+template <typename T>
+__device__ __forceinline__ u32x4 sys_chunk_load(__amdgpu_buffer_rsrc_t r, int64_t base, int64_t e,
+                                                int64_t n) {
+    constexpr int VV = 16 / (int)sizeof(T);
+    u32x4 t = {0u, 0u, 0u, 0u};
+    if (e + VV <= n) {
+        t = __builtin_amdgcn_raw_buffer_load_b128(r, (int)((base + e) * (int64_t)sizeof(T)), 0, 0);
+    } else if (e < n) {
+#pragma unroll
+        for (int v = 0; v < VV; ++v)
+            if (e + v < n)
+                reinterpret_cast<T *>(&t)[v] = bload<T>(r, (int)((base + e + v) * (int64_t)sizeof(T)), 0);
+    }
+    return t;
+}
+template <typename T>
+__device__ __forceinline__ void sys_chunk_store(u32x4 t, __amdgpu_buffer_rsrc_t r, int64_t base,
+                                                int64_t e, int64_t n) {
+    constexpr int VV = 16 / (int)sizeof(T);
+    if (e + VV <= n) {
+        __builtin_amdgcn_raw_buffer_store_b128(t, r, (int)((base + e) * (int64_t)sizeof(T)), 0, 0);
+    } else if (e < n) {
+#pragma unroll
+        for (int v = 0; v < VV; ++v)
+            if (e + v < n)
+                bstore<T>(reinterpret_cast<const T *>(&t)[v], r,
+                          (int)((base + e + v) * (int64_t)sizeof(T)), 0);
+    }
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64) thomas_rows_kernel(const T *a, const T *b, T *c, T *d,
+                                                         int64_t n, int64_t batch, int64_t sstride,
+                                                         int32_t bb) {
+    constexpr int VV = 16 / (int)sizeof(T);
+    constexpr int NV = R / VV;                 // vectors per array and chunk
+    static_assert(R % VV == 0, "chunks are whole 16-byte vectors");
+    const int64_t s = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (s >= batch) return;                    // no LDS or barriers below
+    const __amdgpu_buffer_rsrc_t ra = make_rsrc(a, bb), rb = make_rsrc(b, bb), rc = make_rsrc(c, bb),
+                                 rd = make_rsrc(d, bb);
+    const int64_t base = s * sstride;
+    u32x4 cur[4][NV], nxt[4][NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        cur[0][v] = sys_chunk_load<T>(ra, base, v * VV, n);
+        cur[1][v] = sys_chunk_load<T>(rb, base, v * VV, n);
+        cur[2][v] = sys_chunk_load<T>(rc, base, v * VV, n);
+        cur[3][v] = sys_chunk_load<T>(rd, base, v * VV, n);
+    }
+    T cp = 0, dp = 0;
+    for (int64_t r0 = 0; r0 < n; r0 += R) {
+        const bool more = r0 + R < n;
+        if (more) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                nxt[0][v] = sys_chunk_load<T>(ra, base, r0 + R + v * VV, n);
+                nxt[1][v] = sys_chunk_load<T>(rb, base, r0 + R + v * VV, n);
+                nxt[2][v] = sys_chunk_load<T>(rc, base, r0 + R + v * VV, n);
+                nxt[3][v] = sys_chunk_load<T>(rd, base, r0 + R + v * VV, n);
+            }
+        }
+        T *A = reinterpret_cast<T *>(cur[0]), *B = reinterpret_cast<T *>(cur[1]);
+        T *C = reinterpret_cast<T *>(cur[2]), *D = reinterpret_cast<T *>(cur[3]);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int64_t i = r0 + j;
+            if (i < n) {
+                const T aj = (i == 0) ? T(0) : A[j];          // a_0 is not part of A
+                const T cj = (i == n - 1) ? T(0) : C[j];      // nor is c_{n-1}
+                const T k = rcp(B[j] - aj * cp);
+                cp = cj * k;
+                dp = (D[j] - aj * dp) * k;
+                C[j] = cp;
+                D[j] = dp;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            sys_chunk_store<T>(cur[2][v], rc, base, r0 + v * VV, n);
+            sys_chunk_store<T>(cur[3][v], rd, base, r0 + v * VV, n);
+        }
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) cur[k][v] = nxt[k][v];
+        }
+    }
+    // back substitution, last chunk first; the last chunk's c', d' are still in cur
+    T xn = 0;
+    for (int64_t r0 = ((n - 1) / R) * R; r0 >= 0; r0 -= R) {
+        const bool more = r0 >= R;
+        if (more) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                nxt[2][v] = sys_chunk_load<T>(rc, base, r0 - R + v * VV, n);
+                nxt[3][v] = sys_chunk_load<T>(rd, base, r0 - R + v * VV, n);
+            }
+        }
+        T *C = reinterpret_cast<T *>(cur[2]), *D = reinterpret_cast<T *>(cur[3]);
+#pragma unroll
+        for (int j = R - 1; j >= 0; --j) {
+            const int64_t i = r0 + j;
+            if (i < n) {
+                xn = (i == n - 1) ? D[j] : D[j] - C[j] * xn;
+                D[j] = xn;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sys_chunk_store<T>(cur[3][v], rd, base, r0 + v * VV, n);
+        if (more) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                cur[2][v] = nxt[2][v];
+                cur[3][v] = nxt[3][v];
+            }
+        }
+    }
+}
+
+// Rows per chunk: 64 bytes of each array.
+template <typename T> constexpr int thomas_rows_chunk() { return 64 / (int)sizeof(T); }
+
+template <typename T>
+static hipError_t launch_thomas_rows(const T *a, const T *b, T *c, T *d, int64_t n, int64_t batch,
+                                     int64_t sstride, int32_t bb, hipStream_t st) {
+    constexpr int VV = 16 / (int)sizeof(T);
+    auto al = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    if (n < 1 || bb == 0 || sstride % VV != 0 || !al(a) || !al(b) || !al(c) || !al(d))
+        return hipErrorInvalidValue;
+    if (batch <= 0) return hipSuccess;
+    const int64_t blocks = (batch + 63) / 64;
+    if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((thomas_rows_kernel<T, thomas_rows_chunk<T>()>), dim3((unsigned)blocks),
+                       dim3(64), 0, st, a, b, c, d, n, batch, sstride, bb);
+    return hipGetLastError();
+}
+
+hipError_t launch_thomas_rows_f32(const float *a, const float *b, float *c, float *d, int64_t n,
+                                  int64_t batch, int64_t sstride, int32_t bb, hipStream_t st) {
+    return launch_thomas_rows<float>(a, b, c, d, n, batch, sstride, bb, st);
+}
+hipError_t launch_thomas_rows_f64(const double *a, const double *b, double *c, double *d,
+                                  int64_t n, int64_t batch, int64_t sstride, int32_t bb,
+                                  hipStream_t st) {
+    return launch_thomas_rows<double>(a, b, c, d, n, batch, sstride, bb, st);
+}
+
 int lane_thomas_max_rows(int precision) { return precision == 0 ? 32 : 16; }
 
 hipError_t launch_lane_thomas_f32(const float *a, const float *b, const float *c, float *d,
