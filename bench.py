@@ -71,7 +71,12 @@ def time_solver(torch, T, sets, n, batch, steps, warmup, algo, layout):
             for s in sets]
     for k in range(warmup):
         T.check(L.tridiag_solve(*args[k % len(args)]))
-    _settle(torch, lambda: L.tridiag_solve(*args[0]))
+    turn = [0]
+
+    def rotate():                              # settle on the rotated sets, as timed
+        turn[0] += 1
+        return L.tridiag_solve(*args[turn[0] % len(args)])
+    _settle(torch, rotate)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     rcs = 0
     t0 = time.perf_counter()
@@ -295,6 +300,28 @@ def run_ragged(torch, T, steps, warmup, n_sys=65536, nmin=32, nmax=8192):
     }
     log("ragged", json.dumps(r))
     return r
+
+
+def run_vendor(torch, T, steps, warmup, ours):
+    """The reference's vendor comparator (cuSPARSE gtsv2StridedBatch behind the
+    cuSparse_* symbols, Tridiagonal_CUDA/tridiagonal_basic.cu:205-247) in its ROCm
+    form, hipSPARSE gtsv2StridedBatch (algo id 0), at 4096 x 4096 layout 0, timed
+    the same way as the solve; `ours_speedup` = its kernel time / ours."""
+    import numpy as np
+    out = {}
+    for name, dt in (("f32", np.float32), ("f64", np.float64)):
+        sets = make_sets(torch, dt, N_EQT, N_BATCH, N_SETS, seed0=11, layout=0)
+        k = max(2, min(steps, 20))
+        wall, kern = time_solver(torch, T, sets, N_EQT, N_BATCH, k, min(warmup, 3), 0, 0)
+        del sets
+        torch.cuda.empty_cache()
+        par = check_parity(torch, T, dt, N_EQT, N_BATCH, 0, 0, 1e-5 if dt == np.float32 else 1e-12)
+        mine = ours[name + "_contiguous"]["kernel_ms"]
+        out[name] = {"kernel_ms": kern, "ms_per_step": wall, "steps": k,
+                     "unknowns_per_s": N_EQT * N_BATCH / (kern * 1e-3), "parity": par,
+                     "ours_kernel_ms": mine, "ours_speedup": kern / mine}
+        log("vendor gtsv", name, json.dumps(out[name]))
+    return out
 
 
 def run_convert(torch, T, steps, warmup, n=N_EQT, batch=N_BATCH):
@@ -563,6 +590,8 @@ def main():
     large = run_large(torch, T, args.steps, args.warmup)
     ragged = run_ragged(torch, T, args.steps, args.warmup)
     fwdbwd = run_fwd_bwd(torch, T, args.steps, args.warmup)
+    log("hipSPARSE gtsv2StridedBatch")
+    vendor = run_vendor(torch, T, args.steps, args.warmup, res)
     log("layout conversion")
     convert = run_convert(torch, T, args.steps, args.warmup)
     log("n=1024 x 1024 FP64")
@@ -592,6 +621,7 @@ def main():
         "ragged": ragged,
         "n1024_f64": n1024,
         "layout_convert_4096x4096": convert,
+        "vendor_gtsv2_4096x4096": vendor,
         "fwd_bwd": fwdbwd,
     }
     print(json.dumps(line), flush=True)
