@@ -13,7 +13,9 @@ import sys
 
 from . import build as _b
 
-_FIELDS = [("VGPRs", "vgpr"), ("AGPRs", "agpr"), ("SGPRs", "sgpr"),
+# remark labels (older compilers print "SGPRs", newer ones "TotalSGPRs")
+_FIELDS = [("VGPRs", "vgpr"), ("AGPRs", "agpr"), ("SGPRs", "sgpr"), ("TotalSGPRs", "sgpr"),
+           ("VGPRs Spill", "vspill"), ("SGPRs Spill", "sspill"),
            ("ScratchSize [bytes/lane]", "scratch"), ("Occupancy [waves/SIMD]", "occ"),
            ("LDS Size [bytes/block]", "lds")]
 
@@ -50,12 +52,13 @@ def report(src):
 
 
 def format_rows(rows):
-    out = ["%-6s %-6s %-6s %-8s %-4s %-7s  %s" % ("VGPR", "AGPR", "SGPR", "scratch", "occ",
-                                                   "LDS", "kernel")]
+    out = ["%-6s %-6s %-6s %-7s %-7s %-8s %-4s %-7s  %s" % (
+        "VGPR", "AGPR", "SGPR", "vspill", "sspill", "scratch", "occ", "LDS", "kernel")]
     for r in rows:
-        out.append("%-6s %-6s %-6s %-8s %-4s %-7s  %s" % (
-            r.get("vgpr", "?"), r.get("agpr", "?"), r.get("sgpr", "?"), r.get("scratch", "?"),
-            r.get("occ", "?"), r.get("lds", "?"), r["name"]))
+        out.append("%-6s %-6s %-6s %-7s %-7s %-8s %-4s %-7s  %s" % (
+            r.get("vgpr", "?"), r.get("agpr", "?"), r.get("sgpr", "?"), r.get("vspill", "?"),
+            r.get("sspill", "?"), r.get("scratch", "?"), r.get("occ", "?"), r.get("lds", "?"),
+            r["name"]))
     return "\n".join(out)
 
 
