@@ -49,10 +49,13 @@ IL_SHAPES = {
 C_SHAPES = {
     "float": [(M, NW, S) for (NW, S) in ((4, 4), (4, 2), (4, 1), (8, 1), (16, 1))
               for M in (2, 4, 8, 16, 32) if not (M == 32 and NW == 16)] +
-             [(32, 16, 1), (64, 8, 1)],
+             [(32, 16, 1), (64, 8, 1)] +
+             # 12 rows per lane: capacities of 1.5x a power of two (768 ... 12288)
+             [(12, 4, 4), (12, 4, 2), (12, 4, 1), (12, 8, 1), (12, 16, 1)],
     "double": [(M, NW, S) for (NW, S) in ((4, 4), (4, 2), (4, 1), (8, 1), (16, 1))
                for M in (2, 4, 8, 16) if not (M == 16 and NW == 16)] +
-              [(16, 16, 1), (32, 4, 1), (32, 8, 1)],
+              [(16, 16, 1), (32, 4, 1), (32, 8, 1)] +
+              [(12, 4, 4), (12, 4, 2), (12, 4, 1), (12, 8, 1)],
 }
 ALDS_MIN_M = {"float": 32, "double": 16}
 VEC_W = {"float": 4, "double": 2}
@@ -88,7 +91,9 @@ def _entries(T):
             out.append((M, NW, S, False, vec, alds, False, False, False, False))
             out.append((M, NW, S, False, vec, alds, False, True, False, False))
             if vec and not alds and 64 * M * VEC_W_BYTES[T] // VEC_W[T] <= 8192:
+                # whole-line loads through LDS, exact fit and padded (chunks bounded by n)
                 out.append((M, NW, S, False, vec, alds, False, False, True, True))
+                out.append((M, NW, S, False, vec, alds, False, True, True, True))
     return out
 
 

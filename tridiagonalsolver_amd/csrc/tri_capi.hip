@@ -283,6 +283,22 @@ static int64_t large_group_bytes() {
     return v;
 }
 
+// Contiguous systems above this many rows take the chunked solve even where an
+// on-chip tile holds them ($TRIDIAG_LARGE_MIN_ROWS overrides).  Default: two
+// chunks (16384 rows FP32, 8192 FP64).  Measured on MI355X at 2^24 unknowns: the
+// tiles above that size hold so much state per lane that they ran at 1.3-2.4 TB/s,
+// the chunked solve at 3.1-3.2 TB/s; at or below it the tiles ran 4.4-4.9 TB/s.
+template <typename T> static int64_t large_min_rows() {
+    static const int64_t v = [] {
+        if (const char *e = getenv("TRIDIAG_LARGE_MIN_ROWS")) {
+            const long long x = atoll(e);
+            if (x > 0) return (int64_t)x;
+        }
+        return (int64_t)2 * large_chunk_rows(sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE, 0);
+    }();
+    return v;
+}
+
 // Systems above the on-chip capacity: the three-pass chunked solve, in groups
 // of systems, with stream-ordered scratch for the chunk outer forms.
 template <typename T>
@@ -409,6 +425,9 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
     const bool vec_ok = layout != kInterleaved && (sstride % V) == 0 && aligned16<T>(a) &&
                         aligned16<T>(b) && aligned16<T>(c) && aligned16<T>(d);
     const int32_t bb = buffer_extent<T>(n, batch, sstride, layout);
+    if (layout != kInterleaved && n > large_min_rows<T>() &&
+        n > large_chunk_rows(sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE, 0))
+        return large_impl<T>(algo, n, batch, sstride, a, b, c, d, st);
     const TileEntry<T> *e = choose_tile<T>(algo, layout, n, vec_ok, bb == 0);
     if (!e && layout != kInterleaved && n > max_onchip_rows<T>())
         return large_impl<T>(algo, n, batch, sstride, a, b, c, d, st);
@@ -846,6 +865,10 @@ int tridiag_plan(int precision, int algo, int layout, int64_t n_eqt, int64_t n_b
     (void)n_batch;
     if (algo == TRIDIAG_ALGO_AUTO) algo = kWM5Reg;
     if (algo <= kThomas) return fail(TRIDIAG_EUNSUPPORTED, "no tile plan for this algorithm");
+    const bool chunked = layout != 2 && n_eqt > (precision == TRIDIAG_SINGLE
+                                                     ? large_min_rows<float>()
+                                                     : large_min_rows<double>());
+    if (chunked) return fail(TRIDIAG_EUNSUPPORTED, "n_eqt takes the chunked solve, not a tile");
     int m = 0, nw = 0, s = 0;
     bool found = false;
     if (precision == TRIDIAG_SINGLE) {

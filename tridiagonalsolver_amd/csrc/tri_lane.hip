@@ -168,38 +168,6 @@ static hipError_t launch_lane_thomas(const T *a, const T *b, const T *c, T *d, i
 // loads the next chunk while it eliminates the current one, so a system costs
 // about 2n/R dependent memory round trips instead of 2n.
 
-// Chunk of 16 bytes at row e of the system starting at element `base` of the
-// array behind r; rows at or past n read as zero and are never touched.
-template <typename T>
-__device__ __forceinline__ u32x4 sys_chunk_load(__amdgpu_buffer_rsrc_t r, int64_t base, int64_t e,
-                                                int64_t n) {
-    constexpr int VV = 16 / (int)sizeof(T);
-    u32x4 t = {0u, 0u, 0u, 0u};
-    if (e + VV <= n) {
-        t = __builtin_amdgcn_raw_buffer_load_b128(r, (int)((base + e) * (int64_t)sizeof(T)), 0, 0);
-    } else if (e < n) {
-#pragma unroll
-        for (int v = 0; v < VV; ++v)
-            if (e + v < n)
-                reinterpret_cast<T *>(&t)[v] = bload<T>(r, (int)((base + e + v) * (int64_t)sizeof(T)), 0);
-    }
-    return t;
-}
-template <typename T>
-__device__ __forceinline__ void sys_chunk_store(u32x4 t, __amdgpu_buffer_rsrc_t r, int64_t base,
-                                                int64_t e, int64_t n) {
-    constexpr int VV = 16 / (int)sizeof(T);
-    if (e + VV <= n) {
-        __builtin_amdgcn_raw_buffer_store_b128(t, r, (int)((base + e) * (int64_t)sizeof(T)), 0, 0);
-    } else if (e < n) {
-#pragma unroll
-        for (int v = 0; v < VV; ++v)
-            if (e + v < n)
-                bstore<T>(reinterpret_cast<const T *>(&t)[v], r,
-                          (int)((base + e + v) * (int64_t)sizeof(T)), 0);
-    }
-}
-
 template <typename T, int R>
 __global__ void __launch_bounds__(64) thomas_rows_kernel(const T *a, const T *b, T *c, T *d,
                                                          int64_t n, int64_t batch, int64_t sstride,

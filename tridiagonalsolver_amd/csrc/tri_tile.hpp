@@ -79,7 +79,8 @@ struct TileShape {
     static constexpr int V = (VEC && !IL) ? Vec16<T>::N : 1;      // elements per load
     // Prefetch ring: NBUF chunks of CH rows; at most (NBUF-1)*CH*4/V loads in
     // flight per wave, below the 63 outstanding loads a wave can count.
-    static constexpr int CHR = (V > 1) ? 8 : (sizeof(T) == 4 ? 4 : 2);
+    static constexpr int CHR0 = (V > 1) ? 8 : (sizeof(T) == 4 ? 4 : 2);
+    static constexpr int CHR = (M % CHR0 == 0) ? CHR0 : 4;       // M = 12: chunks of 4 rows
     static constexpr int CH = (M < CHR) ? M : CHR;
     static constexpr int NC = M / CH;
     static constexpr int NBUF_MAX = (NW >= 16) ? 2 : 3;   // 1024-lane groups: 128 VGPRs
@@ -152,6 +153,43 @@ __device__ __forceinline__ void tail_chunk_store(u32x4 t, __amdgpu_buffer_rsrc_t
             if (e + v < n)
                 bstore<T, NT>(reinterpret_cast<const T *>(&t)[v], r,
                               (int)((e + v) * (int64_t)sizeof(T)), 0);
+    }
+}
+
+// 16-byte chunk at row e of the n-row system that starts at element `base` of the
+// array behind r (a buffer below 2 GiB): the whole chunk inside the system, its
+// leading rows when it straddles n, zero past n.  Never touches rows at or past n.
+template <typename T, bool NT = false>
+__device__ __forceinline__ u32x4 sys_chunk_load(__amdgpu_buffer_rsrc_t r, int64_t base, int64_t e,
+                                                int64_t n) {
+    constexpr int VV = 16 / (int)sizeof(T);
+    u32x4 t = {0u, 0u, 0u, 0u};
+    if (e + VV <= n) {
+        t = __builtin_amdgcn_raw_buffer_load_b128(r, (int)((base + e) * (int64_t)sizeof(T)), 0,
+                                                  cpol<NT>());
+    } else if (e < n) {
+#pragma unroll
+        for (int v = 0; v < VV; ++v)
+            if (e + v < n)
+                reinterpret_cast<T *>(&t)[v] =
+                    bload<T, NT>(r, (int)((base + e + v) * (int64_t)sizeof(T)), 0);
+    }
+    return t;
+}
+// Stores the rows of a 16-byte chunk that lie inside the system (see sys_chunk_load).
+template <typename T, bool NT = false>
+__device__ __forceinline__ void sys_chunk_store(u32x4 t, __amdgpu_buffer_rsrc_t r, int64_t base,
+                                                int64_t e, int64_t n) {
+    constexpr int VV = 16 / (int)sizeof(T);
+    if (e + VV <= n) {
+        __builtin_amdgcn_raw_buffer_store_b128(t, r, (int)((base + e) * (int64_t)sizeof(T)), 0,
+                                               cpol<NT>());
+    } else if (e < n) {
+#pragma unroll
+        for (int v = 0; v < VV; ++v)
+            if (e + v < n)
+                bstore<T, NT>(reinterpret_cast<const T *>(&t)[v], r,
+                              (int)((base + e + v) * (int64_t)sizeof(T)), 0);
     }
 }
 
@@ -509,8 +547,8 @@ template <typename T, int M, int NW, int S, bool IL, bool VEC, bool ALDS, bool B
 __global__ void __launch_bounds__(64 * NW)
     __attribute__((amdgpu_waves_per_eu(tile_min_waves<T, M, NW, XP>()))) tile_wm_kernel(TileParams<T> prm) {
     using Sh = TileShape<T, M, NW, S, IL, VEC, ALDS, BIG, PAD, NT, XP>;
-    static_assert(!XP || (!PAD && !BIG && (IL ? (S * (int)sizeof(T)) % 16 == 0 : VEC)),
-                  "transposed loads need exact tiles (and 16-byte row pieces when interleaved)");
+    static_assert(!XP || (!BIG && (IL ? (!PAD && (S * (int)sizeof(T)) % 16 == 0) : VEC)),
+                  "transposed loads need 16-byte rows (and exact tiles when interleaved)");
     constexpr int SW = Sh::SW, NWS = Sh::NWS, LS = Sh::LS, V = Sh::V, CH = Sh::CH;
     constexpr int LANES = 64 * NW;
     const int tid = threadIdx.x;
@@ -596,14 +634,23 @@ __global__ void __launch_bounds__(64 * NW)
             // wave block: 64 segments x M rows = 64*M contiguous elements per array
             constexpr int NQ = 64 * M * (int)sizeof(T) / 1024;   // 16-byte loads per lane
             constexpr int PL = M * (int)sizeof(T) / 16;          // 16-byte pieces per lane row set
+            constexpr int VV = 16 / (int)sizeof(T);
             const int wvo = (int)((base + (int64_t)ws * 64 * M) * (int64_t)sizeof(T)) + lane * 16;
+            const int64_t e0 = (int64_t)ws * 64 * M + lane * VV;   // first row of the lane's chunk
+            (void)wvo;
+            (void)e0;
             u32x4 g[4][NQ];
             const __amdgpu_buffer_rsrc_t rr[4] = {ra, rb, rc, rd_rsrc};
 #pragma unroll
             for (int k = 0; k < 4; ++k)
 #pragma unroll
-                for (int q = 0; q < NQ; ++q)
-                    g[k][q] = __builtin_amdgcn_raw_buffer_load_b128(rr[k], wvo, q * 1024, cpol<NT>());
+                for (int q = 0; q < NQ; ++q) {
+                    if constexpr (PAD)   // chunks bounded by the system's n
+                        g[k][q] = sys_chunk_load<T, NT>(rr[k], base, e0 + q * 64 * VV, n);
+                    else
+                        g[k][q] = __builtin_amdgcn_raw_buffer_load_b128(rr[k], wvo, q * 1024,
+                                                                        cpol<NT>());
+                }
             T rows[4][M];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -618,6 +665,11 @@ __global__ void __launch_bounds__(64 * NW)
                     for (int v = 0; v < 16 / (int)sizeof(T); ++v)
                         rows[k][i * (16 / (int)sizeof(T)) + v] = reinterpret_cast<const T *>(&t)[v];
                 }
+            }
+            if constexpr (PAD) {
+#pragma unroll
+                for (int j = 0; j < M; ++j)   // rows past the system: identity rows
+                    if (row0 + j >= n) { rows[0][j] = 0; rows[1][j] = 1; rows[2][j] = 0; rows[3][j] = 0; }
             }
             RegLoader<T, CH, M> ld{rows};
             lane_sweep<T, Sh>(o, ld, row0, n, al_store);
@@ -706,10 +758,17 @@ __global__ void __launch_bounds__(64 * NW)
         }
         wave_lds_sync();                          // all rows written before the block is read
         const int wvo = (int)((base + (int64_t)ws * 64 * M) * (int64_t)sizeof(T)) + lane * 16;
+        const int64_t e0 = (int64_t)ws * 64 * M + lane * VV;
+        (void)wvo;
+        (void)e0;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
-            __builtin_amdgcn_raw_buffer_store_b128(xbuf[wave][q * 64 + lane], rd_rsrc, wvo, q * 1024,
-                                                   cpol<NT>());
+        for (int q = 0; q < NQ; ++q) {
+            if constexpr (PAD)
+                sys_chunk_store<T, NT>(xbuf[wave][q * 64 + lane], rd_rsrc, base, e0 + q * 64 * VV, n);
+            else
+                __builtin_amdgcn_raw_buffer_store_b128(xbuf[wave][q * 64 + lane], rd_rsrc, wvo,
+                                                       q * 1024, cpol<NT>());
+        }
         return;
     }
     T *__restrict__ D = prm.d;
