@@ -221,16 +221,39 @@ def merge_ragged_table(path, buckets):
         f.write("\n".join(lines) + "\n")
 
 
-def best(results, tol, tie=0.01):
-    """Fastest accurate result; among those within `tie` of it, the one with the
-    fewest waves per workgroup (timing differences that small are run-to-run noise,
-    and smaller workgroups hold fewer registers per lane)."""
+_KEY = ("M", "NW", "S", "vec", "lds_alpha", "nontemporal", "transposed")
+
+
+def read_table(path):
+    """{(precision, layout, n_max): kernel key} of the tile lines of a table file."""
+    out = {}
+    if not path or not os.path.exists(path):
+        return out
+    for ln in open(path):
+        f = ln.split()
+        if not f or f[0].startswith("#") or f[0] == "R" or len(f) != 11:
+            continue
+        v = [int(x) for x in f]
+        out[(v[0], v[1], v[3])] = tuple(v[4:])
+    return out
+
+
+def best(results, tol, tie=0.01, incumbent=None):
+    """Fastest accurate result.  Among those within `tie` of it (run-to-run noise)
+    the table's current choice for the bucket (`incumbent`, a kernel key) is kept;
+    otherwise the one with the fewest waves per system, then per workgroup (a
+    shorter cross-wave interface chain)."""
     ok = [r for r in results if r["relerr"] <= tol]
     if not ok:
         return None
     fastest = min(r["ms"] for r in ok)
     near = [r for r in ok if r["ms"] <= fastest * (1 + tie)]
-    return min(near, key=lambda r: (r["NW"], r["ms"]))
+    for r in near:
+        if incumbent is not None and tuple(int(r[k]) for k in _KEY) == tuple(incumbent):
+            return r
+    # interleaved tiles spread every wave over all S systems
+    per_sys = lambda r: r["NW"] if r.get("interleaved") else r["NW"] / r["S"]
+    return min(near, key=lambda r: (per_sys(r), r["NW"], r["ms"]))
 
 
 def write_table(path, rows):
@@ -254,7 +277,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--prec", type=int, nargs="+", default=[0, 1])
     ap.add_argument("--layout", type=int, nargs="+", default=[2])
-    ap.add_argument("--n", type=int, nargs="+", default=[4096])
+    ap.add_argument("--n", nargs="+", default=["4096"],
+                    help="sizes to tune; N:NMAX tunes at N and records the bucket as n <= NMAX "
+                         "(e.g. 1792:2047 for sizes just below a power of two, on padded kernels)")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--unknowns", type=int, default=0,
                     help="if set, batch = unknowns // n for each n (constant work per point)")
@@ -277,16 +302,22 @@ def main(argv=None):
         _log("done in %.1f s" % (time.time() - t0))
         return
     allres, rows = [], []
+    current = read_table(args.table)
     t0 = time.time()
     for prec in args.prec:
         for layout in args.layout:
-            for n in args.n:
+            for spec in args.n:
+                n, _, nmax = str(spec).partition(":")
+                n = int(n)
+                nmax = int(nmax) if nmax else n
                 batch = args.unknowns // n if args.unknowns else args.batch
                 res = sweep(prec, layout, n, batch, reps=args.reps)
-                allres.append(dict(prec=prec, layout=layout, n=n, batch=batch, results=res))
-                b = best(res, 1e-5 if prec == 0 else 1e-12)
+                allres.append(dict(prec=prec, layout=layout, n=n, n_max=nmax, batch=batch,
+                                   results=res))
+                b = best(res, 1e-5 if prec == 0 else 1e-12,
+                         incumbent=current.get((prec, layout, nmax)))
                 if b:
-                    rows.append((prec, layout, n, b))
+                    rows.append((prec, layout, nmax, b))
                     _log("best prec %d layout %d n %d -> M=%d NW=%d S=%d vec=%d alds=%d nt=%d xp=%d "
                          "%.4f ms" % (prec, layout, n, b["M"], b["NW"], b["S"], b["vec"],
                                       b["lds_alpha"], b["nontemporal"], b["transposed"], b["ms"]))
