@@ -86,3 +86,25 @@ def test_over_limit_and_interleaved_are_rejected():
     with pytest.raises(T.TridiagError) as e:
         T.solve(big, big, big, big, algo=-1, layout=2, n=n, batch=2)
     assert e.value.code == 2
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("frac", [0, 1, 2])
+@pytest.mark.parametrize("algo", [-1, 3, 13])
+def test_chunked_below_onchip_capacity(prec, frac, algo):
+    """Contiguous systems above two chunks take the chunked solve even though an
+    on-chip tile would hold them (16384 < n <= 32768 FP32, 8192 < n <= 16384 FP64)."""
+    lo = 2 * CHUNK[prec]
+    n = [lo + 1, lo + lo // 2 + 3, ONCHIP[prec]][frac]
+    _check(prec, algo, 0, n, 3, seed=n + 11 * frac, blo=0.05, bhi=0.2)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_plan_reports_chunked_sizes(prec):
+    import tridiagonalsolver_amd as T
+    lo = 2 * CHUNK[prec]
+    assert T.plan(prec, -1, 0, lo, 64)["M"] > 0          # two chunks' worth: on-chip tile
+    with pytest.raises(T.TridiagError) as e:
+        T.plan(prec, -1, 0, lo + 1, 64)                   # above: the chunked solve
+    assert e.value.code == 2
+    assert T.plan(prec, -1, 2, 2048, 64)["M"] > 0          # interleaved: tiles only
