@@ -40,10 +40,10 @@ def _to(v):
 def test_adjoint_and_fused(prec, layout, n):
     import torch
     import tridiagonalsolver_amd as T
-    if prec == 1 and n > 4096:
-        pytest.skip("FP64 adjoint tiles hold up to 4096 rows")
-    if layout == 2 and n > 1024:
-        pytest.skip("interleaved adjoint tiles hold up to 1024 rows")
+    # sizes beyond the fused adjoint kernels (4096 rows FP64, 1024 interleaved) take
+    # the composed path: the plain solver on A^T plus element-wise gradients
+    if layout == 2 and n > 4096:
+        pytest.skip("interleaved solves hold up to 4096 rows")
     batch = 37
     dtype = DT[prec]
     announce("adjoint", prec, layout, n)
@@ -65,6 +65,32 @@ def test_adjoint_and_fused(prec, layout, n):
     assert _rel(xo.cpu().numpy().astype(np.float64), x64) <= TOL[prec]
     for k, o in enumerate((ga, gb, gc, gd)):
         assert _rel(o.cpu().numpy().astype(np.float64), ref[k]) <= TOL[prec], k
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("n", [20001, 70000])
+def test_adjoint_long_systems(prec, n):
+    """Long contiguous systems: the composed adjoint over the chunked solve."""
+    import torch
+    import tridiagonalsolver_amd as T
+    batch = 3
+    dtype = DT[prec]
+    announce("adjoint long", prec, n)
+    a, b, c, d = make_case(n, batch, 0, dtype, seed=n, blo=0.05, bhi=0.2)
+    g = np.random.default_rng(n).standard_normal(n * batch).astype(dtype)
+    x64 = oracle.thomas(a, b, c, d, n, batch, layout=0)
+    ref = oracle.adjoint(a, b, c, x64, g, n, batch, layout=0)
+    ta, tb, tc, td, tg = (_to(v) for v in (a, b, c, d, g))
+    xo, ga, gb, gc, gd = T.solve_fwd_bwd(ta, tb, tc, td, tg, layout=0, n=n, batch=batch)
+    outs = T.solve_adjoint(ta, tb, tc, _to(x64.astype(dtype)), tg, layout=0, n=n, batch=batch)
+    torch.cuda.synchronize()
+    assert _rel(xo.cpu().numpy().astype(np.float64), x64) <= TOL[prec]
+    for k, o in enumerate((ga, gb, gc, gd)):
+        assert _rel(o.cpu().numpy().astype(np.float64), ref[k]) <= TOL[prec], ("fused", k)
+    for k, o in enumerate(outs):
+        assert _rel(o.cpu().numpy().astype(np.float64), ref[k]) <= TOL[prec], ("adjoint", k)
+    # inputs are left as they were
+    assert np.array_equal(tc.cpu().numpy(), c) and np.array_equal(td.cpu().numpy(), d)
 
 
 @pytest.mark.parametrize("prec", [0, 1])

@@ -727,10 +727,36 @@ static int adjoint_impl(int mode, int algo, int layout, int64_t n, int64_t batch
     const int64_t sstride = layout == kInterleaved ? 1 : n;
     const int64_t rstride = layout == kInterleaved ? batch : 1;
     const int32_t bb = buffer_extent<T>(n, batch, sstride, layout);
-    if (bb == 0) return fail(TRIDIAG_EUNSUPPORTED, "adjoint arrays must be below 2 GiB");
-    const AdjEntry<T> *e = choose_adj<T>(layout, n, mode, algo);
-    if (!e) return fail(TRIDIAG_EUNSUPPORTED, "n_eqt=" + std::to_string(n) +
-                                                  " exceeds the on-chip capacity of the adjoint");
+    const AdjEntry<T> *e = bb ? choose_adj<T>(layout, n, mode, algo) : nullptr;
+    if (!e) {
+        // Beyond the fused kernels: the shifted coefficients of A^T go into ga / gc and
+        // g into gd, A^T lambda = g is solved in place on gd by the plain solver (x
+        // first, in place on xo, when fused), then the gradients overwrite ga, gb, gc.
+        const bool il = layout == kInterleaved;
+        hipError_t err = (sizeof(T) == 4)
+            ? launch_adj_shift_f32((const float *)a, (const float *)c, (const float *)g,
+                                   mode == kFused ? (const float *)d : nullptr, (float *)ga,
+                                   (float *)gc, (float *)gd, (float *)xo, n, batch, il, st)
+            : launch_adj_shift_f64((const double *)a, (const double *)c, (const double *)g,
+                                   mode == kFused ? (const double *)d : nullptr, (double *)ga,
+                                   (double *)gc, (double *)gd, (double *)xo, n, batch, il, st);
+        if (err != hipSuccess) return hip_fail(err, "adjoint coefficients");
+        const T *xs = x;
+        if (mode == kFused) {
+            const int rc = solve_impl<T>(algo, layout, n, batch, const_cast<T *>(a),
+                                         const_cast<T *>(b), const_cast<T *>(c), xo, st);
+            if (rc != TRIDIAG_OK) return rc;
+            xs = xo;
+        }
+        const int rc = solve_impl<T>(algo, layout, n, batch, ga, const_cast<T *>(b), gc, gd, st);
+        if (rc != TRIDIAG_OK) return rc;
+        err = (sizeof(T) == 4)
+            ? launch_adj_grad_f32((const float *)xs, (const float *)gd, (float *)ga, (float *)gb,
+                                  (float *)gc, n, batch, il, st)
+            : launch_adj_grad_f64((const double *)xs, (const double *)gd, (double *)ga,
+                                  (double *)gb, (double *)gc, n, batch, il, st);
+        return err == hipSuccess ? TRIDIAG_OK : hip_fail(err, "adjoint gradients");
+    }
     const AlgoSpec sp = algo_spec(algo);
     AdjParams<T> p{a, b, c, d, x, g, xo, ga, gb, gc, gd, n, batch, sstride, rstride, bb,
                    sp.iface, sp.xchg_lds};

@@ -90,6 +90,19 @@ hipError_t launch_thomas_rows_f32(const float *a, const float *b, float *c, floa
 hipError_t launch_thomas_rows_f64(const double *a, const double *b, double *c, double *d,
                                   int64_t n, int64_t batch, int64_t sstride, int32_t bb,
                                   hipStream_t st);
+// Composed adjoint (systems the fused adjoint kernels do not hold): the shifted
+// coefficients of A^T into ap / cp, g into lam, and d into xo when d is given;
+// then, after the solves, the coefficient gradients from x and lambda.
+hipError_t launch_adj_shift_f32(const float *a, const float *c, const float *g, const float *d,
+                                float *ap, float *cp, float *lam, float *xo, int64_t n,
+                                int64_t batch, bool il, hipStream_t st);
+hipError_t launch_adj_shift_f64(const double *a, const double *c, const double *g,
+                                const double *d, double *ap, double *cp, double *lam, double *xo,
+                                int64_t n, int64_t batch, bool il, hipStream_t st);
+hipError_t launch_adj_grad_f32(const float *x, const float *lam, float *ga, float *gb, float *gc,
+                               int64_t n, int64_t batch, bool il, hipStream_t st);
+hipError_t launch_adj_grad_f64(const double *x, const double *lam, double *ga, double *gb,
+                               double *gc, int64_t n, int64_t batch, bool il, hipStream_t st);
 hipError_t launch_thomas_f32(const float *a, const float *b, float *c, float *d, int64_t n,
                              int64_t batch, int64_t sstride, int64_t rstride, hipStream_t st);
 hipError_t launch_thomas_f64(const double *a, const double *b, double *c, double *d, int64_t n,
