@@ -42,8 +42,6 @@ def test_adjoint_and_fused(prec, layout, n):
     import tridiagonalsolver_amd as T
     # sizes beyond the fused adjoint kernels (4096 rows FP64, 1024 interleaved) take
     # the composed path: the plain solver on A^T plus element-wise gradients
-    if layout == 2 and n > 4096:
-        pytest.skip("interleaved solves hold up to 4096 rows")
     batch = 37
     dtype = DT[prec]
     announce("adjoint", prec, layout, n)

@@ -73,7 +73,7 @@ def test_packed_layout_fp32():
     _check(0, 14, 1, ONCHIP[0] + 1000, 3, seed=78)
 
 
-def test_over_limit_and_interleaved_are_rejected():
+def test_over_limit_is_rejected():
     import torch
     import tridiagonalsolver_amd as T
     n = 1025 * CHUNK[1]
@@ -81,11 +81,14 @@ def test_over_limit_and_interleaved_are_rejected():
     with pytest.raises(T.TridiagError) as e:
         T.solve(big, big, big, big, algo=-1, layout=0, n=n, batch=1)
     assert e.value.code == 2
-    n = ONCHIP[1] + 1
-    big = torch.zeros(n * 2, dtype=torch.float64, device="cuda")
-    with pytest.raises(T.TridiagError) as e:
-        T.solve(big, big, big, big, algo=-1, layout=2, n=n, batch=2)
-    assert e.value.code == 2
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("n", [4097, 8191, 20000, 70001])
+def test_long_interleaved_systems(prec, n):
+    """Interleaved systems longer than the interleaved tiles hold: converted to the
+    contiguous layout, solved there (chunked when long) and converted back."""
+    _check(prec, -1, 2, n, 5, seed=n, blo=0.05, bhi=0.2)
 
 
 @pytest.mark.parametrize("prec", [0, 1])

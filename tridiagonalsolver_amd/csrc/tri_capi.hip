@@ -339,6 +339,39 @@ static int large_impl(int algo, int64_t n, int64_t batch, int64_t sstride, T *a,
     return rc;
 }
 
+template <typename T>
+static int convert_impl(int src_layout, int dst_layout, int64_t n, int64_t batch,
+                        const T *const src[4], T *const dst[4], hipStream_t st);
+template <typename T>
+static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b, T *c, T *d,
+                      hipStream_t st);
+
+// Interleaved systems longer than the interleaved tiles hold: converted to the
+// contiguous layout in stream-ordered scratch, solved there (the chunked solve for
+// long n), and x converted back.  Moves 8w + solve + 2w bytes per unknown.
+template <typename T>
+static int interleaved_via_contiguous(int algo, int64_t n, int64_t batch, T *a, T *b, T *c, T *d,
+                                      hipStream_t st) {
+    const int64_t elems = n * batch;
+    void *scratch = nullptr;
+    hipError_t e = hipMallocAsync(&scratch, (size_t)(4 * elems) * sizeof(T), st);
+    if (e != hipSuccess) return hip_fail(e, "interleaved solve scratch");
+    T *w = reinterpret_cast<T *>(scratch);
+    const T *src[4] = {a, b, c, d};
+    T *dst[4] = {w, w + elems, w + 2 * elems, w + 3 * elems};
+    int rc = convert_impl<T>(kInterleaved, kContiguous, n, batch, src, dst, st);
+    if (rc == TRIDIAG_OK) rc = solve_impl<T>(algo, kContiguous, n, batch, dst[0], dst[1], dst[2],
+                                             dst[3], st);
+    if (rc == TRIDIAG_OK) {
+        const T *xs[4] = {nullptr, nullptr, nullptr, dst[3]};
+        T *xd[4] = {nullptr, nullptr, nullptr, d};
+        rc = convert_impl<T>(kContiguous, kInterleaved, n, batch, xs, xd, st);
+    }
+    const hipError_t ef = hipFreeAsync(scratch, st);
+    if (rc == TRIDIAG_OK && ef != hipSuccess) rc = hip_fail(ef, "interleaved solve scratch");
+    return rc;
+}
+
 // Largest n for which the automatic choice takes the one-system-per-lane Thomas
 // kernel instead of an on-chip tile ($TRIDIAG_LANE_ROWS overrides).
 template <typename T> static int64_t lane_auto_rows(int layout) {
@@ -431,6 +464,8 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
     const TileEntry<T> *e = choose_tile<T>(algo, layout, n, vec_ok, bb == 0);
     if (!e && layout != kInterleaved && n > max_onchip_rows<T>())
         return large_impl<T>(algo, n, batch, sstride, a, b, c, d, st);
+    if (!e && layout == kInterleaved)      // no interleaved tile takes it
+        return interleaved_via_contiguous<T>(algo, n, batch, a, b, c, d, st);
     if (!e) {
         return fail(TRIDIAG_EUNSUPPORTED,
                     "n_eqt=" + std::to_string(n) + " exceeds the on-chip capacity of algorithm " +
@@ -768,6 +803,7 @@ static int adjoint_impl(int mode, int algo, int layout, int64_t n, int64_t batch
 using namespace tri;
 
 // ======================================================================= Part 2
+namespace tri {
 // Layout conversion.  Element (coefficient k, system s, row i) sits at
 // ptr_k[s*n + i] (layout 0), buf[s*4n + k*n + i] (layout 1) or ptr_k[i*batch + s]
 // (layout 2); layouts 0 and 1 are row-pitched views of the same system-major
@@ -835,6 +871,7 @@ static int convert_impl(int src_layout, int dst_layout, int64_t n, int64_t batch
     else e = launch_layout_move_f64(p, count, transpose, st);
     return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "layout conversion launch");
 }
+}  // namespace tri
 
 extern "C" {
 
