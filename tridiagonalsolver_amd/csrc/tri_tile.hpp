@@ -121,41 +121,6 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, int32
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
 }
 
-// 16-byte chunk starting at element e (16/sizeof(T) elements) of a row range of
-// n elements addressed by r: the whole chunk when it lies inside the range, its
-// leading elements when it straddles the end, zero past the end.  Never touches
-// memory outside the range.
-template <typename T, bool NT>
-__device__ __forceinline__ u32x4 tail_chunk_load(__amdgpu_buffer_rsrc_t r, int64_t e, int64_t n) {
-    constexpr int VV = 16 / (int)sizeof(T);
-    u32x4 t = {0u, 0u, 0u, 0u};
-    if (e + VV <= n) {
-        t = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(e * (int64_t)sizeof(T)), 0, cpol<NT>());
-    } else if (e < n) {
-#pragma unroll
-        for (int v = 0; v < VV; ++v)
-            if (e + v < n)
-                reinterpret_cast<T *>(&t)[v] = bload<T, NT>(r, (int)((e + v) * (int64_t)sizeof(T)), 0);
-    }
-    return t;
-}
-
-// Stores the part of a 16-byte chunk that lies inside the n-element range.
-template <typename T, bool NT>
-__device__ __forceinline__ void tail_chunk_store(u32x4 t, __amdgpu_buffer_rsrc_t r, int64_t e,
-                                                 int64_t n) {
-    constexpr int VV = 16 / (int)sizeof(T);
-    if (e + VV <= n) {
-        __builtin_amdgcn_raw_buffer_store_b128(t, r, (int)(e * (int64_t)sizeof(T)), 0, cpol<NT>());
-    } else if (e < n) {
-#pragma unroll
-        for (int v = 0; v < VV; ++v)
-            if (e + v < n)
-                bstore<T, NT>(reinterpret_cast<const T *>(&t)[v], r,
-                              (int)((e + v) * (int64_t)sizeof(T)), 0);
-    }
-}
-
 // 16-byte chunk at row e of the n-row system that starts at element `base` of the
 // array behind r (a buffer below 2 GiB): the whole chunk inside the system, its
 // leading rows when it straddles n, zero past n.  Never touches rows at or past n.
@@ -191,6 +156,17 @@ __device__ __forceinline__ void sys_chunk_store(u32x4 t, __amdgpu_buffer_rsrc_t 
                 bstore<T, NT>(reinterpret_cast<const T *>(&t)[v], r,
                               (int)((base + e + v) * (int64_t)sizeof(T)), 0);
     }
+}
+
+// The same for a buffer that starts at the first row of the range (base 0).
+template <typename T, bool NT>
+__device__ __forceinline__ u32x4 tail_chunk_load(__amdgpu_buffer_rsrc_t r, int64_t e, int64_t n) {
+    return sys_chunk_load<T, NT>(r, 0, e, n);
+}
+template <typename T, bool NT>
+__device__ __forceinline__ void tail_chunk_store(u32x4 t, __amdgpu_buffer_rsrc_t r, int64_t e,
+                                                 int64_t n) {
+    sys_chunk_store<T, NT>(t, r, 0, e, n);
 }
 
 __device__ __forceinline__ int64_t wave_uniform(int64_t x) {
