@@ -147,6 +147,28 @@ WORKLOADS = [
 ]
 
 
+def reference_style_ms(torch, T, sets, n, batch, reps):
+    """Host clock around the synchronous reference entry point (WM5_reg_single /
+    WM5_reg_double: default stream, returns after the solve), averaged over `reps`
+    calls on the rotated sets, as the reference driver times a solver
+    (Tridiagonal_CUDA/test.cpp:94-100).  Includes launch and synchronisation."""
+    import ctypes
+    prec = 0 if sets[0][0].dtype == torch.float32 else 1
+    fn = getattr(T.lib(), "WM5_reg_single" if prec == 0 else "WM5_reg_double")
+    fn.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int]
+    fn.restype = None
+    ptrs = [[ctypes.c_void_p(t.data_ptr()) for t in s] for s in sets]
+    for k in range(3):
+        fn(*ptrs[k % len(ptrs)], n, batch)
+    torch.cuda.synchronize()
+    t = 0.0
+    for k in range(reps):
+        t0 = time.perf_counter()
+        fn(*ptrs[k % len(ptrs)], n, batch)
+        t += time.perf_counter() - t0
+    return t * 1e3 / reps
+
+
 def run_workload(torch, T, name, dtname, layout, steps, warmup):
     import numpy as np
     dtype = np.dtype(dtname).type
@@ -155,6 +177,9 @@ def run_workload(torch, T, name, dtname, layout, steps, warmup):
     sets = make_sets(torch, dtype, n, batch, N_SETS, seed0=11, layout=layout)
     log("time", name)
     wall, kern = time_solver(torch, T, sets, n, batch, steps, warmup, -1, layout)
+    ref_style = None
+    if layout == 0:
+        ref_style = reference_style_ms(torch, T, sets, n, batch, steps)
     del sets
     torch.cuda.empty_cache()
     w = np.dtype(dtype).itemsize
@@ -165,6 +190,7 @@ def run_workload(torch, T, name, dtname, layout, steps, warmup):
     par = check_parity(torch, T, dtype, n, batch, -1, layout, 1e-5 if w == 4 else 1e-12)
     achieved = alg_bytes / (kern * 1e-3) / 1e9
     r = dict(ms_per_step=wall, kernel_ms=kern, unknowns_per_s=n * batch / (wall * 1e-3),
+             reference_style_sync_ms=ref_style,
              achieved_gbs=achieved, alg_bytes=alg_bytes, plan=pl, parity=par,
              layout={0: "contiguous", 2: "interleaved"}[layout],
              roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -615,7 +641,8 @@ def main():
         "roofline": main_r["roofline"],
         "cpu_baseline": cpu,
         "workloads": {k: {kk: v[kk] for kk in ("unknowns_per_s", "ms_per_step", "kernel_ms",
-                                               "layout", "plan", "parity", "roofline")}
+                                               "reference_style_sync_ms", "layout", "plan",
+                                               "parity", "roofline")}
                       for k, v in res.items()},
         "large": large,
         "ragged": ragged,
