@@ -245,3 +245,25 @@ def test_full_size_workload():
     announce("ragged full size", int(sizes.sum()))
     off, arrs = _case(sizes, np.float32, seed=2025)
     _check(off, arrs, _run(off, arrs), np.float32)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_misaligned_arrays(prec):
+    """Arrays that start one row past a 16-byte boundary (views into larger buffers):
+    systems read on the aligned chunk grid must not reach before the array."""
+    import torch
+    import tridiagonalsolver_amd as T
+    dtype = np.float32 if prec == 0 else np.float64
+    sizes = oracle.ragged_sizes(900, 1, 3000, seed=41)
+    off, arrs = _case(sizes, dtype, seed=42)
+    dev = torch.device("cuda:0")
+    big = [torch.zeros(len(v) + 1, dtype=torch.float32 if prec == 0 else torch.float64,
+                       device=dev) for v in arrs]
+    for t, v in zip(big, arrs):
+        t[1:] = torch.from_numpy(v).to(dev)
+    views = [t[1:] for t in big]
+    plan = T.RaggedPlan(off, dtype=views[0].dtype, device=dev)
+    plan.solve(*views)
+    torch.cuda.synchronize()
+    assert float(big[3][0]) == 0.0                       # nothing written before the array
+    _check(off, arrs, views[3].cpu().numpy(), dtype)
