@@ -258,10 +258,20 @@ def best(results, tol, tie=0.01, incumbent=None):
 
 def write_table(path, rows):
     """rows: list of (prec, layout, n_max, entry) tuples.  Ragged ("R") lines of an
-    existing table are kept."""
+    existing table are kept, and so are its tile lines of (precision, layout) pairs
+    that `rows` does not cover."""
     os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
-    ragged = [ln for ln in open(path)] if os.path.exists(path) else []
-    ragged = [ln.rstrip("\n") for ln in ragged if ln.startswith("R ")]
+    old = [ln.rstrip("\n") for ln in open(path)] if os.path.exists(path) else []
+    ragged = [ln for ln in old if ln.startswith("R ")]
+    tuned = {(r[0], r[1]) for r in rows}
+    for ln in old:
+        f = ln.split()
+        if len(f) == 11 and not f[0].startswith("#") and f[0] != "R" and \
+                (int(f[0]), int(f[1])) not in tuned:
+            v = [int(x) for x in f]
+            rows = rows + [(v[0], v[1], v[3], dict(M=v[4], NW=v[5], S=v[6], vec=v[7],
+                                                   lds_alpha=v[8], nontemporal=v[9],
+                                                   transposed=v[10]))]
     with open(path, "w") as f:
         f.write("# precision layout family n_max M NW S vec lds_alpha nt xp  "
                 "(written by autotune.py)\n")
@@ -310,7 +320,8 @@ def main(argv=None):
                 n, _, nmax = str(spec).partition(":")
                 n = int(n)
                 nmax = int(nmax) if nmax else n
-                batch = args.unknowns // n if args.unknowns else args.batch
+                # a multiple of 64 systems keeps interleaved rows line-aligned
+                batch = max(64, args.unknowns // n // 64 * 64) if args.unknowns else args.batch
                 res = sweep(prec, layout, n, batch, reps=args.reps)
                 allres.append(dict(prec=prec, layout=layout, n=n, n_max=nmax, batch=batch,
                                    results=res))

@@ -216,6 +216,12 @@ template <typename T, int CH, int V, bool NT = false> struct PredLoader {
     int vo, rowb;
     int64_t row0, n;
     __device__ __forceinline__ void load(T (&q)[4][CH], int j0) const {
+        if (row0 + j0 + CH <= n) {
+            // the whole chunk lies inside the system (every chunk but one per system):
+            // the unpredicated loads of FastLoader, without a branch per row
+            FastLoader<T, CH, V, NT>{ra, rb, rc, rd, vo, rowb}.load(q, j0);
+            return;
+        }
 #pragma unroll
         for (int v0 = 0; v0 < CH; v0 += V) {
             const int64_t r = row0 + j0 + v0;
@@ -748,6 +754,28 @@ __global__ void __launch_bounds__(64 * NW)
         return;
     }
     T *__restrict__ D = prm.d;
+    if (!BIG && !fast && row0 + M <= n) {
+        // padded tiles: a segment wholly inside the system (all but one per system)
+        // stores without a predicate per row
+#pragma unroll
+        for (int j0 = 0; j0 < M; j0 += V) {
+            T xv[V];
+#pragma unroll
+            for (int vv = 0; vv < V; ++vv) {
+                const int j = j0 + vv;
+                xv[vv] = (j == 0) ? xL : ((j == M - 1) ? xR : de[(j >= 1 && j <= M - 2) ? j - 1 : 0]);
+            }
+            if (V > 1) {
+                u32x4 xs;
+#pragma unroll
+                for (int vv = 0; vv < V; ++vv) reinterpret_cast<T *>(&xs)[vv] = xv[vv];
+                __builtin_amdgcn_raw_buffer_store_b128(xs, rd_rsrc, vo, j0 * rowb, cpol<NT>());
+            } else {
+                bstore<T, NT>(xv[0], rd_rsrc, vo, j0 * rowb);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int j0 = 0; j0 < M; j0 += V) {
         T xv[V];
