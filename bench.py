@@ -627,7 +627,8 @@ def main():
         log("cpu baseline")
         cpu = cpu_baseline()
     line = {
-        "metric": "unknowns/s & HBM % of peak (4096^2); ragged+fwd/bwd vs plain @relerr<=1e-5",
+        "metric": "unknowns/s & HBM % of peak (4096\u00b2); ragged+fwd/bwd vs plain "
+                  "@relerr\u22641e-5",
         "value": main_r["unknowns_per_s"], "unit": "unknowns/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "settle_ms": SETTLE_MS,
         "ms_per_step": main_r["ms_per_step"],
