@@ -111,3 +111,12 @@ def test_plan_reports_chunked_sizes(prec):
         T.plan(prec, -1, 0, lo + 1, 64)                   # above: the chunked solve
     assert e.value.code == 2
     assert T.plan(prec, -1, 2, 2048, 64)["M"] > 0          # interleaved: tiles only
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("algo,n", [(11, 4097), (11, 6000), (12, 8193), (13, 16385), (11, 10638)])
+def test_fixed_rows_algorithms_above_their_tiles(prec, algo, n):
+    """CR_reg R fixes R rows per lane, so its tiles hold at most 64*16*R rows; longer
+    contiguous systems take the chunked solve (one chunk or more) with its CR
+    interface family instead of failing."""
+    _check(prec, algo, 0, n, 3, seed=algo * 1000 + n)

@@ -462,7 +462,7 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
         n > large_chunk_rows(sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE, 0))
         return large_impl<T>(algo, n, batch, sstride, a, b, c, d, st);
     const TileEntry<T> *e = choose_tile<T>(algo, layout, n, vec_ok, bb == 0);
-    if (!e && layout != kInterleaved && n > max_onchip_rows<T>())
+    if (!e && layout != kInterleaved)   // no tile of this algorithm holds n (e.g. CR_reg R)
         return large_impl<T>(algo, n, batch, sstride, a, b, c, d, st);
     if (!e && layout == kInterleaved)      // no interleaved tile takes it
         return interleaved_via_contiguous<T>(algo, n, batch, a, b, c, d, st);
