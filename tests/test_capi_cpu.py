@@ -62,6 +62,10 @@ def test_argument_validation_without_gpu(capi):
     assert L.tridiag_solve(0, 9, 0, 16, 4, nul, nul, nul, nul, nul) == capi.EINVAL
     assert b"null" in L.tridiag_last_error()
     assert L.tridiag_solve(0, 9, 5, 16, 4, nul, nul, nul, nul, nul) == capi.EINVAL
+    # the vendor comparator (algo 0) needs layout 0 and at least 3 rows
+    assert L.tridiag_solve(0, 0, 2, 16, 4, nul, nul, nul, nul, nul) == capi.EUNSUPPORTED
+    assert L.tridiag_solve(1, 0, 0, 2, 4, nul, nul, nul, nul, nul) == capi.EUNSUPPORTED
+    assert b">= 3" in L.tridiag_last_error()
 
 
 @pytest.mark.parametrize("prec", [0, 1])

@@ -900,6 +900,8 @@ int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_
         return fail(TRIDIAG_EINVAL, "unknown algorithm id " + std::to_string(algo));
     if (algo == kVendor) {
         if (layout != kContiguous) return fail(TRIDIAG_EUNSUPPORTED, "gtsv needs layout 0");
+        if (n_eqt < 3)
+            return fail(TRIDIAG_EUNSUPPORTED, "the vendor gtsv2StridedBatch needs n_eqt >= 3");
         if (n_eqt > 0x7fffffff || n_batch > 0x7fffffff) return fail(TRIDIAG_EINVAL, "size");
         int rc = (precision == TRIDIAG_SINGLE)
                      ? sparse_init<float>((float *)a, (float *)b, (float *)c, (float *)d,
