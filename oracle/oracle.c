@@ -426,6 +426,7 @@ OR_EXPORT void oracle_gen_dd_ragged(double *a, double *b, double *c, double *d,
     for (int64_t s = 0; s < nsys; ++s) {
         uint64_t state = oracle_sm64_stream_seed(seed, s);
         int64_t o = offsets[s], n = offsets[s + 1] - offsets[s];
+        if (n <= 0) continue;            /* empty system: no rows to write */
         double *A = a + o, *B = b + o, *C = c + o, *D = d + o;
         for (int64_t i = 0; i < n; ++i) A[i] = sm64_unif(&state, -1.0, 1.0);
         for (int64_t i = 0; i < n; ++i) C[i] = sm64_unif(&state, -1.0, 1.0);

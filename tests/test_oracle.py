@@ -332,3 +332,19 @@ def test_adjoint_central_differences(oracle_mod):
             p[k][i] += h; m[k][i] -= h
             fd = (w @ oracle_mod.thomas(*p, n, 1) - w @ oracle_mod.thomas(*m, n, 1)) / (2 * h)
             assert grads[k][i] == pytest.approx(fd, rel=1e-6, abs=1e-8)
+
+
+def test_ragged_generator_stays_inside_empty_end_systems(oracle_mod):
+    """Empty first and last systems: the generator writes nothing outside the rows
+    (guard elements on both sides of every array stay untouched)."""
+    import ctypes
+    sizes = np.array([0, 4, 0, 3, 0], dtype=np.int64)
+    off = oracle_mod.offsets_from_sizes(sizes)
+    total = int(off[-1])
+    bufs = [np.full(total + 2, 7.0) for _ in range(4)]          # guard, rows, guard
+    ptrs = [b[1:].ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for b in bufs]
+    offp = off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    oracle_mod.lib().oracle_gen_dd_ragged(*ptrs, offp, len(sizes), 5, 1.0, 2.0, 1)
+    for bf in bufs:
+        assert bf[0] == 7.0 and bf[-1] == 7.0
+        assert np.all(bf[1:-1] != 7.0)                           # every row was written
