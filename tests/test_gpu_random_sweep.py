@@ -88,3 +88,37 @@ def test_random_adjoints(chunk):
             if not err <= TOL[dtype]:
                 fails.append((prec, layout, n, batch, k, err))
     assert not fails, fails
+
+
+def test_random_ragged_batches():
+    """Random ragged batches: sizes uniform in [0, n_max] (empty systems anywhere,
+    the first and last included), both precisions, every system against the oracle."""
+    import torch
+    import tridiagonalsolver_amd as T
+    rng = np.random.default_rng(4242)
+    fails = []
+    for it in range(12):
+        prec = int(rng.integers(0, 2))
+        dtype = DT[prec]
+        nsys = int(rng.integers(1, 2000))
+        nmax = int(rng.choice([16, 100, 1000, 8192]))
+        sizes = rng.integers(0, nmax + 1, nsys)
+        sizes[0] = 0
+        sizes[-1] = 0
+        off = oracle.offsets_from_sizes(sizes.astype(np.int64))
+        announce("sweep ragged", prec, nsys, nmax)
+        a, b, c, d = (v.astype(dtype) for v in oracle.gen_dd_ragged(off, seed=it))
+        xr = oracle.thomas_ragged(*(v.astype(np.float64) for v in (a, b, c, d)), off)
+        ts = [torch.from_numpy(v).cuda() for v in (a, b, c, d)]
+        if int(off[-1]) == 0:
+            continue
+        T.RaggedPlan(off, dtype=ts[0].dtype, device="cuda").solve(*ts)
+        torch.cuda.synchronize()
+        x = ts[3].cpu().numpy().astype(np.float64)
+        for s in np.nonzero(sizes)[0]:
+            sl = slice(off[s], off[s + 1])
+            err = np.max(np.abs(x[sl] - xr[sl])) / max(np.max(np.abs(xr[sl])), 1e-300)
+            if not err <= TOL[dtype]:
+                fails.append((it, int(s), float(err)))
+                break
+    assert not fails, fails
