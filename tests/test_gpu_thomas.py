@@ -1,7 +1,9 @@
-"""GPU parity of Thomas (algo 1) in the contiguous and packed layouts, where each
-lane walks its own system in 16-byte row chunks with the next chunk in flight:
-lengths with partial last chunks, odd lengths (element path), batches that leave
-lanes idle, against the FP64 oracle: normwise relative error <= 1e-5 / 1e-12."""
+"""GPU parity of Thomas (algo 1) in the contiguous and packed layouts, where the
+lanes of a group share one system and load its rows several steps ahead: lengths
+with partial last vectors and steps, odd lengths (element path), lengths at the
+step and prefetch-ring boundaries, batches that leave groups idle, against the FP64
+oracle: normwise relative error <= 1e-5 / 1e-12.  c holds c' afterwards, as in the
+reference (Tridiagonal_CUDA/tridiagonal_basic.cu:4-30)."""
 import numpy as np
 import pytest
 
@@ -42,3 +44,42 @@ def test_thomas_low_margin_long(prec):
     x = run_solve(arrs, n, batch, 0, 1, DT[prec])
     st, res = check_against_oracle(arrs, x, n, batch, 0, DT[prec])
     assert st["nonfinite"] == 0 and st["relerr"] <= TOL[DT[prec]], st
+
+
+# rows per step: 16 lanes x one 16-byte vector; prefetch rings of 4 steps (forward)
+# and 8 steps (backward)
+_STEP = {0: 64, 1: 32}
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_thomas_step_boundaries(prec):
+    sr = _STEP[prec]
+    sizes = sorted({17, 33, sr - 1, sr, sr + 1, 2 * sr + 3, 4 * sr - 1, 4 * sr, 4 * sr + 1,
+                    8 * sr - 1, 8 * sr, 8 * sr + 1, 13 * sr + 5})
+    for n in sizes:
+        for batch in (1, 9, 130):
+            announce("thomas steps", prec, n, batch)
+            arrs = make_case(n, batch, 0, DT[prec], seed=n * 7 + batch)
+            x = run_solve(arrs, n, batch, 0, 1, DT[prec])
+            st, res = check_against_oracle(arrs, x, n, batch, 0, DT[prec])
+            assert st["nonfinite"] == 0 and st["relerr"] <= TOL[DT[prec]], (n, batch, st)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_thomas_leaves_c_prime_in_c(prec):
+    import torch
+    import tridiagonalsolver_amd as T
+    n, batch = 300, 20
+    announce("thomas c prime", prec, n, batch)
+    a, b, c, d = make_case(n, batch, 0, DT[prec], seed=5)
+    ts = [torch.from_numpy(v.copy()).cuda() for v in (a, b, c, d)]
+    T.solve(*ts, algo=1, layout=0, n=n, batch=batch)
+    torch.cuda.synchronize()
+    cg = ts[2].cpu().numpy().reshape(batch, n).astype(np.float64)
+    A, B, C = (v.reshape(batch, n).astype(np.float64) for v in (a, b, c))
+    cp = np.zeros((batch, n))
+    cp[:, 0] = C[:, 0] / B[:, 0]
+    for i in range(1, n):
+        cp[:, i] = C[:, i] / (B[:, i] - A[:, i] * cp[:, i - 1])
+    err = np.max(np.abs(cg[:, :-1] - cp[:, :-1])) / np.max(np.abs(cp[:, :-1]))
+    assert err <= TOL[DT[prec]], err

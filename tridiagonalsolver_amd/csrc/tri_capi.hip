@@ -438,7 +438,7 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
     }
     if (algo == kThomas) {
         if (layout != kInterleaved && lane_bb != 0) {
-            // rows of each lane's system as 16-byte chunks (aligned rows below 2 GiB)
+            // lane groups per system, rows as 16-byte vectors (aligned rows below 2 GiB)
             hipError_t e = (sizeof(T) == 4)
                 ? launch_thomas_rows_f32((const float *)a, (const float *)b, (float *)c, (float *)d,
                                          n, batch, sstride, lane_bb, st)

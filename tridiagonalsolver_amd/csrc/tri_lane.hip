@@ -161,119 +161,179 @@ static hipError_t launch_lane_thomas(const T *a, const T *b, const T *c, T *d, i
 }
 
 // ---------------------------------------------------------------- longer systems
-// Thomas over systems of any length in the contiguous (and packed) layout, one
-// system per lane, with the reference's memory semantics: c' is written to c and
-// d' to d on the way down, x to d on the way up (9 words per unknown).  Each lane
-// walks its own rows in chunks of R rows read and written as 16-byte vectors, and
-// loads the next chunk while it eliminates the current one, so a system costs
-// about 2n/R dependent memory round trips instead of 2n.
+// Thomas over systems of any length in the contiguous (and packed) layout, with
+// the reference's memory semantics: c' is written to c and d' to d on the way
+// down, x to d on the way up (9 words per unknown).  The recurrence of a system
+// is serial, so its speed is set by how many of its rows are in flight: a lane
+// that walked its system alone would wait one memory round trip (~1 us) per few
+// rows.  Here L lanes share a system.  A step covers SR = L * 16/w rows: lane t of
+// the system's group loads the t-th 16-byte vector of each array, the group's
+// vectors pass through LDS to all L lanes, and every lane of the group runs the
+// same recurrence over the SR rows, keeping and storing only its own vector of
+// c', d' (and later x).  Loads run DF steps ahead on the way down and DB steps
+// ahead on the way up.  Each row is stored and read back by the same lane.
 
-template <typename T, int R>
-__global__ void __launch_bounds__(64) thomas_rows_kernel(const T *a, const T *b, T *c, T *d,
+template <typename T, int L, int DF, int DB>
+__global__ void __launch_bounds__(64) thomas_team_kernel(const T *a, const T *b, T *c, T *d,
                                                          int64_t n, int64_t batch, int64_t sstride,
                                                          int32_t bb) {
-    constexpr int VV = 16 / (int)sizeof(T);
-    constexpr int NV = R / VV;                 // vectors per array and chunk
-    static_assert(R % VV == 0, "chunks are whole 16-byte vectors");
-    const int64_t s = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (s >= batch) return;                    // no LDS or barriers below
-    const __amdgpu_buffer_rsrc_t ra = make_rsrc(a, bb), rb = make_rsrc(b, bb), rc = make_rsrc(c, bb),
-                                 rd = make_rsrc(d, bb);
-    const int64_t base = s * sstride;
-    u32x4 cur[4][NV], nxt[4][NV];
+    constexpr int VV = 16 / (int)sizeof(T);    // rows per 16-byte vector
+    constexpr int G = 64 / L;                  // systems per wave
+    constexpr int SR = L * VV;                 // rows per step
+    static_assert(64 % L == 0, "a wave holds whole groups");
+    __shared__ u32x4 xs[G][4][L];
+    const int lane = threadIdx.x, g = lane / L, t = lane % L;
+    const int64_t s = (int64_t)blockIdx.x * G + g;
+    const bool valid = s < batch;
+    const int64_t nl = valid ? n : 0;          // rows this lane may touch (none when idle)
+    const int64_t base = valid ? s * sstride : 0;
+    const __amdgpu_buffer_rsrc_t rr[4] = {make_rsrc(a, bb), make_rsrc(b, bb), make_rsrc(c, bb),
+                                          make_rsrc(d, bb)};
+    const int64_t steps = (n + SR - 1) / SR;
+    const int64_t own = (int64_t)t * VV;       // first row of the lane's vector in a step
+    // Vector loads take no bounds branch: a branch ends in a copy that waits for the
+    // newest load and so drains the prefetch ring.  sstride is a multiple of VV and
+    // at least n, so the vector holding row n-1 ends inside the system's stride;
+    // vectors outside [0, n), and idle lanes, read the array's first vector instead,
+    // which is never used.
+    auto vload = [&](int r, int64_t e) -> u32x4 {
+        const int off = ((uint64_t)e < (uint64_t)nl) ? (int)((base + e) * (int64_t)sizeof(T)) : 0;
+        return __builtin_amdgcn_raw_buffer_load_b128(rr[r], off, 0, cpol<false>());
+    };
+
+    // ---- forward elimination
+    // Slots are issued in the order they are consumed (the scheduler would otherwise
+    // reorder the independent loads, and waiting for the first slot would then wait
+    // for all of them).
+    u32x4 ring[DF][4];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        cur[0][v] = sys_chunk_load<T>(ra, base, v * VV, n);
-        cur[1][v] = sys_chunk_load<T>(rb, base, v * VV, n);
-        cur[2][v] = sys_chunk_load<T>(rc, base, v * VV, n);
-        cur[3][v] = sys_chunk_load<T>(rd, base, v * VV, n);
+    for (int k = 0; k < DF; ++k) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ring[k][r] = vload(r, k * SR + own);
+        __builtin_amdgcn_sched_barrier(0);
     }
     T cp = 0, dp = 0;
-    for (int64_t r0 = 0; r0 < n; r0 += R) {
-        const bool more = r0 + R < n;
-        if (more) {
+    for (int64_t q0 = 0; q0 < steps; q0 += DF) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                nxt[0][v] = sys_chunk_load<T>(ra, base, r0 + R + v * VV, n);
-                nxt[1][v] = sys_chunk_load<T>(rb, base, r0 + R + v * VV, n);
-                nxt[2][v] = sys_chunk_load<T>(rc, base, r0 + R + v * VV, n);
-                nxt[3][v] = sys_chunk_load<T>(rd, base, r0 + R + v * VV, n);
-            }
-        }
-        T *A = reinterpret_cast<T *>(cur[0]), *B = reinterpret_cast<T *>(cur[1]);
-        T *C = reinterpret_cast<T *>(cur[2]), *D = reinterpret_cast<T *>(cur[3]);
+        for (int k = 0; k < DF; ++k) {
+            // Every step of the group hands over its slot and refills it, also steps
+            // past the end: a refill skipped on some path would leave the compiler
+            // unable to count the loads in flight, and it would then wait for all of
+            // them.  Only the elimination and the stores are conditional.
+            const int64_t q = q0 + k;
+            const int64_t r0 = q * SR;
+            wave_lds_sync();                   // the group has read the previous step
 #pragma unroll
-        for (int j = 0; j < R; ++j) {
-            const int64_t i = r0 + j;
-            if (i < n) {
-                const T aj = (i == 0) ? T(0) : A[j];          // a_0 is not part of A
-                const T cj = (i == n - 1) ? T(0) : C[j];      // nor is c_{n-1}
-                const T k = rcp(B[j] - aj * cp);
-                cp = cj * k;
-                dp = (D[j] - aj * dp) * k;
-                C[j] = cp;
-                D[j] = dp;
-            }
-        }
+            for (int r = 0; r < 4; ++r) xs[g][r][t] = ring[k][r];
+            wave_lds_sync();
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            sys_chunk_store<T>(cur[2][v], rc, base, r0 + v * VV, n);
-            sys_chunk_store<T>(cur[3][v], rd, base, r0 + v * VV, n);
-        }
-        if (more) {
+            for (int r = 0; r < 4; ++r) ring[k][r] = vload(r, (q + DF) * SR + own);
+            if (q >= steps) continue;
+            u32x4 oc = {0u, 0u, 0u, 0u}, od = oc;
+            // GUARD: the step holds row 0, row n-1 or rows past n.  Interior steps run
+            // without branches, so the LDS reads of the next vector and the selects
+            // fill the latency of the reciprocal chain.
+            auto eliminate = [&](auto guard) {
+                constexpr bool GUARD = decltype(guard)::value;
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
+                for (int u = 0; u < L; ++u) {
+                    const u32x4 va = xs[g][0][u], vb = xs[g][1][u];
+                    u32x4 vc = xs[g][2][u], vd = xs[g][3][u];
+                    const T *A = reinterpret_cast<const T *>(&va);
+                    const T *B = reinterpret_cast<const T *>(&vb);
+                    T *C = reinterpret_cast<T *>(&vc), *D = reinterpret_cast<T *>(&vd);
 #pragma unroll
-                for (int v = 0; v < NV; ++v) cur[k][v] = nxt[k][v];
+                    for (int v = 0; v < VV; ++v) {
+                        const int64_t i = r0 + u * VV + v;
+                        if (!GUARD || i < n) {
+                            const T aj = (GUARD && i == 0) ? T(0) : A[v];      // a_0 is not part of A
+                            const T cj = (GUARD && i == n - 1) ? T(0) : C[v];  // nor is c_{n-1}
+                            const T k2 = rcp(B[v] - aj * cp);
+                            cp = cj * k2;
+                            dp = (D[v] - aj * dp) * k2;
+                            C[v] = cp;
+                            D[v] = dp;
+                        }
+                    }
+                    if (u == t) { oc = vc; od = vd; }
+                }
+            };
+            if (r0 > 0 && r0 + SR < n)
+                eliminate(std::integral_constant<bool, false>{});
+            else
+                eliminate(std::integral_constant<bool, true>{});
+            sys_chunk_store<T>(oc, rr[2], base, r0 + own, nl);
+            sys_chunk_store<T>(od, rr[3], base, r0 + own, nl);
         }
     }
-    // back substitution, last chunk first; the last chunk's c', d' are still in cur
+
+    // ---- back substitution, last step first (c', d' read back by the lanes that
+    // stored them)
+    u32x4 back[DB][2];
+#pragma unroll
+    for (int k = 0; k < DB; ++k) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) back[k][r] = vload(2 + r, (steps - 1 - k) * SR + own);
+        __builtin_amdgcn_sched_barrier(0);
+    }
     T xn = 0;
-    for (int64_t r0 = ((n - 1) / R) * R; r0 >= 0; r0 -= R) {
-        const bool more = r0 >= R;
-        if (more) {
+    for (int64_t q0 = steps - 1; q0 >= 0; q0 -= DB) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                nxt[2][v] = sys_chunk_load<T>(rc, base, r0 - R + v * VV, n);
-                nxt[3][v] = sys_chunk_load<T>(rd, base, r0 - R + v * VV, n);
-            }
-        }
-        T *C = reinterpret_cast<T *>(cur[2]), *D = reinterpret_cast<T *>(cur[3]);
+        for (int k = 0; k < DB; ++k) {
+            const int64_t q = q0 - k;          // unconditional hand-over and refill, as above
+            const int64_t r0 = q * SR;
+            wave_lds_sync();
 #pragma unroll
-        for (int j = R - 1; j >= 0; --j) {
-            const int64_t i = r0 + j;
-            if (i < n) {
-                xn = (i == n - 1) ? D[j] : D[j] - C[j] * xn;
-                D[j] = xn;
-            }
-        }
+            for (int r = 0; r < 2; ++r) xs[g][2 + r][t] = back[k][r];
+            wave_lds_sync();
 #pragma unroll
-        for (int v = 0; v < NV; ++v) sys_chunk_store<T>(cur[3][v], rd, base, r0 + v * VV, n);
-        if (more) {
+            for (int r = 0; r < 2; ++r) back[k][r] = vload(2 + r, (q - DB) * SR + own);
+            if (q < 0) continue;
+            u32x4 ox = {0u, 0u, 0u, 0u};
+            // GUARD: the step holds row n-1 or rows past n
+            auto substitute = [&](auto guard) {
+                constexpr bool GUARD = decltype(guard)::value;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                cur[2][v] = nxt[2][v];
-                cur[3][v] = nxt[3][v];
-            }
+                for (int u = L - 1; u >= 0; --u) {
+                    const u32x4 vc = xs[g][2][u];
+                    u32x4 vd = xs[g][3][u];
+                    const T *C = reinterpret_cast<const T *>(&vc);
+                    T *D = reinterpret_cast<T *>(&vd);
+#pragma unroll
+                    for (int v = VV - 1; v >= 0; --v) {
+                        const int64_t i = r0 + u * VV + v;
+                        if (!GUARD || i < n) {
+                            xn = (GUARD && i == n - 1) ? D[v] : D[v] - C[v] * xn;
+                            D[v] = xn;
+                        }
+                    }
+                    if (u == t) ox = vd;
+                }
+            };
+            if (r0 + SR < n)
+                substitute(std::integral_constant<bool, false>{});
+            else
+                substitute(std::integral_constant<bool, true>{});
+            sys_chunk_store<T>(ox, rr[3], base, r0 + own, nl);
         }
     }
 }
-
-// Rows per chunk: 64 bytes of each array.
-template <typename T> constexpr int thomas_rows_chunk() { return 64 / (int)sizeof(T); }
 
 template <typename T>
 static hipError_t launch_thomas_rows(const T *a, const T *b, T *c, T *d, int64_t n, int64_t batch,
                                      int64_t sstride, int32_t bb, hipStream_t st) {
     constexpr int VV = 16 / (int)sizeof(T);
+    constexpr int L = 16;                      // lanes per system
     auto al = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
-    if (n < 1 || bb == 0 || sstride % VV != 0 || !al(a) || !al(b) || !al(c) || !al(d))
+    // the kernel's unguarded vector loads rely on sstride >= n, a multiple of VV
+    if (n < 1 || bb == 0 || sstride < n || sstride % VV != 0 || !al(a) || !al(b) || !al(c) ||
+        !al(d))
         return hipErrorInvalidValue;
     if (batch <= 0) return hipSuccess;
-    const int64_t blocks = (batch + 63) / 64;
+    const int64_t blocks = (batch + 64 / L - 1) / (64 / L);
     if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((thomas_rows_kernel<T, thomas_rows_chunk<T>()>), dim3((unsigned)blocks),
-                       dim3(64), 0, st, a, b, c, d, n, batch, sstride, bb);
+    hipLaunchKernelGGL((thomas_team_kernel<T, L, 4, 8>), dim3((unsigned)blocks), dim3(64), 0, st,
+                       a, b, c, d, n, batch, sstride, bb);
     return hipGetLastError();
 }
 
