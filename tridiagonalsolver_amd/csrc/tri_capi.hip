@@ -707,7 +707,7 @@ static int ragged_plan_impl(int prec, int64_t n_sys, const int64_t *off, int for
             }
         }
     }
-    RaggedHeader h{kRaggedMagic, n_sys, nitems, 0, prec, 0};
+    RaggedHeader h{kRaggedMagic, n_sys, nitems, 0, prec, 0, n_sys > 0 ? off[n_sys] : 0};
     memcpy(host.data(), &h, sizeof h);
     const size_t used = (size_t)(kRaggedHeaderBytes + (int64_t)kRaggedRecInts * 4 * nitems);
     hipError_t e = hipMemcpyAsync(work, host.data(), used, hipMemcpyHostToDevice, st);

@@ -47,6 +47,7 @@ struct RaggedHeader {
     int32_t counter;     // work-queue head
     int32_t precision;
     int32_t done;        // workgroups of the running solve that have left the queue
+    int64_t rows;        // offsets[n_sys]: rows of the arrays that the systems cover
 };
 constexpr int64_t kRaggedMagic = 0x52414747454432LL;
 constexpr int64_t kRaggedHeaderBytes = 64;
@@ -70,7 +71,7 @@ template <typename T> struct RaggedParams {
 // the lanes through LDS; shorter segments are read row by row.
 template <typename T, int M, int NWS, int S>
 __device__ __forceinline__ void ragged_item(const RaggedParams<T> &p, const int32_t *rec,
-                                            int iface, int count, u32x4 *xbuf,
+                                            int iface, int count, int64_t rows, u32x4 *xbuf,
                                             char *iface_lds) {
     constexpr int VV = 16 / (int)sizeof(T);
     constexpr bool XPL = M * (int)sizeof(T) >= 16;          // whole-chunk loads
@@ -107,13 +108,35 @@ __device__ __forceinline__ void ragged_item(const RaggedParams<T> &p, const int3
     auto al_store = [&](int j, T v) { al_r[j] = v; };
     SweepOut<T, Sh::NI> o;
     if constexpr (XPL) {
-        const __amdgpu_buffer_rsrc_t rr[4] = {ra, rb, rc, rd};
+        // Whole chunks are read without a bounds branch, from resources that reach to
+        // the last row of the arrays: a branch per chunk ends in a copy that waits for
+        // the load in flight, which serialised the loads of every padded system.
+        // Chunks past the system read rows of the next systems, which become identity
+        // rows below.  Only a chunk that holds rows of the system and runs past the
+        // last row of the arrays takes element loads, once all others are in flight.
+        // (idle slots: an empty range, whose loads return zeros without reading memory)
+        const int64_t rem = wave_uniform(!valid ? 0
+                                         : (rows - off < (int64_t)kRaggedMaxN) ? rows - off
+                                                                                : kRaggedMaxN);
+        const int32_t nr = (int32_t)(rem * (int64_t)sizeof(T));
+        const __amdgpu_buffer_rsrc_t rr[4] = {make_rsrc(p.a + off, nr), make_rsrc(p.b + off, nr),
+                                              make_rsrc(p.c + off, nr), make_rsrc(p.d + off, nr)};
         u32x4 g[4][NQ];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
-            for (int q = 0; q < NQ; ++q)
-                g[k][q] = tail_chunk_load<T, true>(rr[k], wrow + (int64_t)(q * 64 + lane) * VV, n);
+            for (int q = 0; q < NQ; ++q) {
+                const int64_t e = wrow + (int64_t)(q * 64 + lane) * VV;
+                const int vo = (e + VV <= rem) ? (int)(e * (int64_t)sizeof(T)) : 0;
+                g[k][q] = __builtin_amdgcn_raw_buffer_load_b128(rr[k], vo, 0, cpol<true>());
+            }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int64_t e = wrow + (int64_t)(q * 64 + lane) * VV;
+                if (e < n && e + VV > rem) g[k][q] = tail_chunk_load<T, true>(rr[k], e, n);
+            }
         T rows[4][M];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -193,6 +216,7 @@ __global__ void __launch_bounds__(kRaggedLanes) ragged_kernel(RaggedParams<T> p)
     const RaggedHeader *h = p.hdr;
     if (h->magic != kRaggedMagic || h->n_sys != p.n_sys || h->precision != p.precision) return;
     const int nitems = (int)h->nitems;
+    const int64_t rows = h->rows;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     u32x4 *xbuf = reinterpret_cast<u32x4 *>(tri_dyn_lds) +
                   (size_t)wave * (64 * kRaggedMaxM * sizeof(T) / 16);
@@ -234,7 +258,7 @@ __global__ void __launch_bounds__(kRaggedLanes) ragged_kernel(RaggedParams<T> p)
 #define TRI_RAGGED_CASE(K)                                                                   \
     case K:                                                                                \
         ragged_item<T, kRaggedShapeTab[K].M, kRaggedShapeTab[K].NWS, kRaggedShapeTab[K].S>( \
-            p, rec, iface, count, xbuf, iface_lds);                                        \
+            p, rec, iface, count, rows, xbuf, iface_lds);                                  \
         break;
             TRI_RAGGED_CASE(0) TRI_RAGGED_CASE(1) TRI_RAGGED_CASE(2) TRI_RAGGED_CASE(3)
             TRI_RAGGED_CASE(4) TRI_RAGGED_CASE(5) TRI_RAGGED_CASE(6) TRI_RAGGED_CASE(7)
