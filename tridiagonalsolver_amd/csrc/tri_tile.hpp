@@ -627,9 +627,16 @@ __global__ void __launch_bounds__(64 * NW)
             for (int k = 0; k < 4; ++k)
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) {
-                    if constexpr (PAD)   // chunks bounded by the system's n
-                        g[k][q] = sys_chunk_load<T, NT>(rr[k], base, e0 + q * 64 * VV, n);
-                    else
+                    if constexpr (PAD) {
+                        // No bounds branch: its copy would wait for the load in flight
+                        // and serialise the loads.  A chunk that holds a row below n
+                        // ends inside the system (the stride is a multiple of VV and at
+                        // least n); chunks at or past n read the system's first chunk
+                        // and become identity rows below.
+                        const int64_t e = e0 + q * 64 * VV;
+                        const int off = (int)((base + (e < n ? e : 0)) * (int64_t)sizeof(T));
+                        g[k][q] = __builtin_amdgcn_raw_buffer_load_b128(rr[k], off, 0, cpol<NT>());
+                    } else
                         g[k][q] = __builtin_amdgcn_raw_buffer_load_b128(rr[k], wvo, q * 1024,
                                                                         cpol<NT>());
                 }
