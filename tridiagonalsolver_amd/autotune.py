@@ -258,16 +258,17 @@ def best(results, tol, tie=0.01, incumbent=None):
 
 def write_table(path, rows):
     """rows: list of (prec, layout, n_max, entry) tuples.  Ragged ("R") lines of an
-    existing table are kept, and so are its tile lines of (precision, layout) pairs
-    that `rows` does not cover."""
+    existing table are kept, and so are its tile lines of buckets (precision, layout,
+    n_max) that `rows` does not cover, so that a sweep over some sizes retunes only
+    their buckets."""
     os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
     old = [ln.rstrip("\n") for ln in open(path)] if os.path.exists(path) else []
     ragged = [ln for ln in old if ln.startswith("R ")]
-    tuned = {(r[0], r[1]) for r in rows}
+    tuned = {(r[0], r[1], r[2]) for r in rows}
     for ln in old:
         f = ln.split()
         if len(f) == 11 and not f[0].startswith("#") and f[0] != "R" and \
-                (int(f[0]), int(f[1])) not in tuned:
+                (int(f[0]), int(f[1]), int(f[3])) not in tuned:
             v = [int(x) for x in f]
             rows = rows + [(v[0], v[1], v[3], dict(M=v[4], NW=v[5], S=v[6], vec=v[7],
                                                    lds_alpha=v[8], nontemporal=v[9],

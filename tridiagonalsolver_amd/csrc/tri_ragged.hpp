@@ -111,9 +111,10 @@ __device__ __forceinline__ void ragged_item(const RaggedParams<T> &p, const int3
         // Whole chunks are read without a bounds branch, from resources that reach to
         // the last row of the arrays: a branch per chunk ends in a copy that waits for
         // the load in flight, which serialised the loads of every padded system.
-        // Chunks past the system read rows of the next systems, which become identity
-        // rows below.  Only a chunk that holds rows of the system and runs past the
-        // last row of the arrays takes element loads, once all others are in flight.
+        // Chunks past the system read the system's first chunk again (a cache hit; its
+        // rows become identity rows below).  Only a chunk that holds rows of the system
+        // and runs past the last row of the arrays takes element loads, once all others
+        // are in flight.
         // (idle slots: an empty range, whose loads return zeros without reading memory)
         const int64_t rem = wave_uniform(!valid ? 0
                                          : (rows - off < (int64_t)kRaggedMaxN) ? rows - off
@@ -127,7 +128,7 @@ __device__ __forceinline__ void ragged_item(const RaggedParams<T> &p, const int3
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 const int64_t e = wrow + (int64_t)(q * 64 + lane) * VV;
-                const int vo = (e + VV <= rem) ? (int)(e * (int64_t)sizeof(T)) : 0;
+                const int vo = (e < n && e + VV <= rem) ? (int)(e * (int64_t)sizeof(T)) : 0;
                 g[k][q] = __builtin_amdgcn_raw_buffer_load_b128(rr[k], vo, 0, cpol<true>());
             }
 #pragma unroll
