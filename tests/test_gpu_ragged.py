@@ -103,6 +103,27 @@ def _shapes():
 
 
 @pytest.mark.parametrize("prec", [0, 1])
+def test_last_system_ends_mid_chunk(prec):
+    """Arrays whose last row is not the end of a 16-byte chunk: the chunk holding the
+    last system's final rows runs past the arrays and is read row by row, every other
+    chunk whole (also chunks past a system, which re-read its first chunk).  Every
+    shape that holds the last system, and the automatic plan."""
+    dtype = DT[prec]
+    vv = 16 // np.dtype(dtype).itemsize
+    for last in (1, 2, 3, 5, 37, 1001, 4097):
+        for lead in ([], [64], [7, 300]):
+            sizes = lead + [last]
+            if sum(sizes) % vv == 0:
+                sizes = lead + [last + 1]
+            off, arrs = _case(np.array(sizes, dtype=np.int64), dtype, seed=last + len(lead))
+            announce("ragged last system", prec, sizes)
+            _check(off, arrs, _run(off, arrs), dtype)
+            for sh in _shapes():
+                if sh["cap"] >= max(sizes):
+                    _check(off, arrs, _run(off, arrs, shape=sh["index"]), dtype)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("iface", [0, 1, 2, 3, 4])
 def test_forced_shape_and_interface(prec, iface):
     """Every shape of the persistent kernel with every interface solver."""
