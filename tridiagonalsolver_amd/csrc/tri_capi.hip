@@ -198,8 +198,10 @@ static const TileEntry<T> *choose_tile(int algo, int layout, int64_t n, bool vec
     if (!fM && !big) {
         std::call_once(g_tune_once, load_tuning);
         const int prec = sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE;
+        // packed systems are solved as contiguous ones (system stride 4n): same rows
+        const int tl = layout == kPacked ? kContiguous : layout;
         for (const TuneRow &r : g_tune) {
-            if (r.prec != prec || r.layout != layout || r.family != 0 || n > r.nmax) continue;
+            if (r.prec != prec || r.layout != tl || r.family != 0 || n > r.nmax) continue;
             if (r.vec && !vec_ok) continue;
             const int nws = il ? r.NW : r.NW / r.S;
             const int64_t cap = (int64_t)(il ? 64 / r.S : 64) * nws * r.M;
