@@ -225,6 +225,11 @@ int tridiag_tile_info(int precision, int index, int *M, int *NW, int *S, int *in
                       int *nontemporal, int *transposed);
 int tridiag_solve_tile(int precision, int index, int layout, int64_t n_eqt, int64_t n_batch,
                        void *a, void *b, void *c, void *d, void *stream);
+/* The same with the interface solver chosen: 0 Wang-Mou merges (what
+ * tridiag_solve_tile uses), 1 Wang-Mou half, 2 PCR, 3 PCR + Thomas, 4 CR.  The
+ * autotuner times each algorithm family on its own interface solver with it. */
+int tridiag_solve_tile_iface(int precision, int index, int iface, int layout, int64_t n_eqt,
+                             int64_t n_batch, void *a, void *b, void *c, void *d, void *stream);
 
 #ifdef __cplusplus
 }

@@ -30,7 +30,8 @@ EXTENDED_SYMBOLS = [
     "tridiag_algo_count", "tridiag_algo_name", "tridiag_algo_layout", "tridiag_solve",
     "tridiag_ragged_work_size", "tridiag_solve_ragged", "tridiag_adjoint",
     "tridiag_solve_fwd_bwd", "tridiag_last_error", "tridiag_plan", "tridiag_tile_count",
-    "tridiag_tile_info", "tridiag_solve_tile", "tridiag_ragged_shape_count",
+    "tridiag_tile_info", "tridiag_solve_tile", "tridiag_solve_tile_iface",
+    "tridiag_ragged_shape_count",
     "tridiag_ragged_shape_info", "tridiag_ragged_bucket", "tridiag_ragged_plan",
     "tridiag_ragged_solve", "tridiag_convert_layout",
 ]
@@ -83,6 +84,8 @@ def lib():
     L.tridiag_tile_info.restype = ci
     L.tridiag_solve_tile.argtypes = [ci, ci, ci, i64, i64, vp, vp, vp, vp, vp]
     L.tridiag_solve_tile.restype = ci
+    L.tridiag_solve_tile_iface.argtypes = [ci, ci, ci, ci, i64, i64, vp, vp, vp, vp, vp]
+    L.tridiag_solve_tile_iface.restype = ci
     L.tridiag_ragged_shape_count.argtypes = []
     L.tridiag_ragged_shape_count.restype = ci
     L.tridiag_ragged_shape_info.argtypes = [ci, pi, pi, pi, ctypes.POINTER(i64)]

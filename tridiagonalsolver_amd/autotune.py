@@ -67,7 +67,7 @@ def _reference(sets, prec, layout, n, batch):
     return x
 
 
-def sweep(prec, layout, n, batch, reps=20, warmup=3, nsets=3, seed=1):
+def sweep(prec, layout, n, batch, reps=20, warmup=3, nsets=3, seed=1, iface=0):
     import torch
     L = _capi.lib()
     tiles = [t for t in _capi.tiles(prec) if t["interleaved"] == (layout == 2)
@@ -87,8 +87,8 @@ def sweep(prec, layout, n, batch, reps=20, warmup=3, nsets=3, seed=1):
         ptrs = [[ctypes.c_void_p(x.data_ptr()) for x in s] for s in sets]
 
         def run(k):
-            return L.tridiag_solve_tile(prec, t["index"], layout, n, batch, *ptrs[k % nsets],
-                                        ctypes.c_void_p(stream.cuda_stream))
+            return L.tridiag_solve_tile_iface(prec, t["index"], iface, layout, n, batch,
+                                              *ptrs[k % nsets], ctypes.c_void_p(stream.cuda_stream))
         rc = run(0)
         if rc != 0:
             _log("skip", t, L.tridiag_last_error().decode())
@@ -225,7 +225,7 @@ _KEY = ("M", "NW", "S", "vec", "lds_alpha", "nontemporal", "transposed")
 
 
 def read_table(path):
-    """{(precision, layout, n_max): kernel key} of the tile lines of a table file."""
+    """{(precision, layout, n_max, family): kernel key} of the tile lines of a table file."""
     out = {}
     if not path or not os.path.exists(path):
         return out
@@ -234,7 +234,7 @@ def read_table(path):
         if not f or f[0].startswith("#") or f[0] == "R" or len(f) != 11:
             continue
         v = [int(x) for x in f]
-        out[(v[0], v[1], v[3])] = tuple(v[4:])
+        out[(v[0], v[1], v[3], v[2])] = tuple(v[4:])
     return out
 
 
@@ -257,31 +257,33 @@ def best(results, tol, tie=0.01, incumbent=None):
 
 
 def write_table(path, rows):
-    """rows: list of (prec, layout, n_max, entry) tuples.  Ragged ("R") lines of an
-    existing table are kept, and so are its tile lines of buckets (precision, layout,
-    n_max) that `rows` does not cover, so that a sweep over some sizes retunes only
-    their buckets."""
+    """rows: list of (prec, layout, n_max, entry) tuples; entry["family"] is the
+    interface solver the entry was tuned with (0 when absent).  Ragged ("R") lines of
+    an existing table are kept, and so are its tile lines of buckets (precision,
+    layout, family, n_max) that `rows` does not cover, so that a sweep over some sizes
+    or families retunes only their buckets."""
     os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
     old = [ln.rstrip("\n") for ln in open(path)] if os.path.exists(path) else []
     ragged = [ln for ln in old if ln.startswith("R ")]
-    tuned = {(r[0], r[1], r[2]) for r in rows}
+    tuned = {(r[0], r[1], r[3].get("family", 0), r[2]) for r in rows}
     for ln in old:
         f = ln.split()
         if len(f) == 11 and not f[0].startswith("#") and f[0] != "R" and \
-                (int(f[0]), int(f[1]), int(f[3])) not in tuned:
+                (int(f[0]), int(f[1]), int(f[2]), int(f[3])) not in tuned:
             v = [int(x) for x in f]
             rows = rows + [(v[0], v[1], v[3], dict(M=v[4], NW=v[5], S=v[6], vec=v[7],
                                                    lds_alpha=v[8], nontemporal=v[9],
-                                                   transposed=v[10]))]
+                                                   transposed=v[10], family=v[2]))]
     with open(path, "w") as f:
         f.write("# precision layout family n_max M NW S vec lds_alpha nt xp  "
                 "(written by autotune.py)\n")
         for ln in ragged:
             f.write(ln + "\n")
-        for prec, layout, nmax, e in sorted(rows, key=lambda r: (r[0], r[1], r[2])):
-            f.write("%d %d 0 %d %d %d %d %d %d %d %d\n" % (
-                prec, layout, nmax, e["M"], e["NW"], e["S"], int(e["vec"]), int(e["lds_alpha"]),
-                int(e["nontemporal"]), int(e["transposed"])))
+        for prec, layout, nmax, e in sorted(rows, key=lambda r: (r[0], r[1], r[3].get("family", 0),
+                                                               r[2])):
+            f.write("%d %d %d %d %d %d %d %d %d %d %d\n" % (
+                prec, layout, e.get("family", 0), nmax, e["M"], e["NW"], e["S"], int(e["vec"]),
+                int(e["lds_alpha"]), int(e["nontemporal"]), int(e["transposed"])))
 
 
 def main(argv=None):
@@ -295,6 +297,10 @@ def main(argv=None):
     ap.add_argument("--unknowns", type=int, default=0,
                     help="if set, batch = unknowns // n for each n (constant work per point)")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--iface", type=int, nargs="+", default=[0],
+                    help="interface solvers to tune with (0 Wang-Mou, the default family; "
+                         "1 Wang-Mou half, 2 PCR, 3 PCR + Thomas, 4 CR): rows of family IFACE, "
+                         "used by the algorithms of that interface solver")
     ap.add_argument("--out", default=None)
     ap.add_argument("--table", default=None, help="write the winners as a tuning table")
     ap.add_argument("--ragged", action="store_true",
@@ -317,22 +323,25 @@ def main(argv=None):
     t0 = time.time()
     for prec in args.prec:
         for layout in args.layout:
-            for spec in args.n:
-                n, _, nmax = str(spec).partition(":")
-                n = int(n)
-                nmax = int(nmax) if nmax else n
-                # a multiple of 64 systems keeps interleaved rows line-aligned
-                batch = max(64, args.unknowns // n // 64 * 64) if args.unknowns else args.batch
-                res = sweep(prec, layout, n, batch, reps=args.reps)
-                allres.append(dict(prec=prec, layout=layout, n=n, n_max=nmax, batch=batch,
-                                   results=res))
-                b = best(res, 1e-5 if prec == 0 else 1e-12,
-                         incumbent=current.get((prec, layout, nmax)))
-                if b:
-                    rows.append((prec, layout, nmax, b))
-                    _log("best prec %d layout %d n %d -> M=%d NW=%d S=%d vec=%d alds=%d nt=%d xp=%d "
-                         "%.4f ms" % (prec, layout, n, b["M"], b["NW"], b["S"], b["vec"],
-                                      b["lds_alpha"], b["nontemporal"], b["transposed"], b["ms"]))
+            for fam in args.iface:
+                for spec in args.n:
+                    n, _, nmax = str(spec).partition(":")
+                    n = int(n)
+                    nmax = int(nmax) if nmax else n
+                    # a multiple of 64 systems keeps interleaved rows line-aligned
+                    batch = max(64, args.unknowns // n // 64 * 64) if args.unknowns else args.batch
+                    res = sweep(prec, layout, n, batch, reps=args.reps, iface=fam)
+                    allres.append(dict(prec=prec, layout=layout, family=fam, n=n, n_max=nmax,
+                                       batch=batch, results=res))
+                    b = best(res, 1e-5 if prec == 0 else 1e-12,
+                             incumbent=current.get((prec, layout, nmax, fam)))
+                    if b:
+                        b = dict(b, family=fam)
+                        rows.append((prec, layout, nmax, b))
+                        _log("best prec %d layout %d family %d n %d -> M=%d NW=%d S=%d vec=%d "
+                             "alds=%d nt=%d xp=%d %.4f ms" % (
+                                 prec, layout, fam, n, b["M"], b["NW"], b["S"], b["vec"],
+                                 b["lds_alpha"], b["nontemporal"], b["transposed"], b["ms"]))
     if args.out:
         with open(args.out, "w") as f:
             json.dump(allres, f, indent=1)

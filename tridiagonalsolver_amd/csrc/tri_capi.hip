@@ -161,14 +161,17 @@ static int forced_M(int algo) {
 }
 
 // LDS one workgroup of `e` needs with the interface solver of `algo`.
-template <typename T> static long lds_need(const TileEntry<T> &e, int algo) {
+template <typename T> static long lds_need_iface(const TileEntry<T> &e, int iface, int xchg_lds) {
     const long w = sizeof(T), lanes = 64L * e.NW;
     const long nws = e.IL ? e.NW : e.NW / e.S;
     long st = (e.ALDS ? (long)(e.M - 2) * lanes * w : 0) + (e.XP ? lanes * e.M * w : 0) +
               (long)e.S * 2 * nws * 8 * w;
-    const AlgoSpec sp = algo_spec(algo);
-    long dyn = sp.iface >= kIfacePCR ? 8 * lanes * w : (sp.xchg_lds ? 6 * lanes * w : 0);
+    long dyn = iface >= kIfacePCR ? 8 * lanes * w : (xchg_lds ? 6 * lanes * w : 0);
     return st + dyn;
+}
+template <typename T> static long lds_need(const TileEntry<T> &e, int algo) {
+    const AlgoSpec sp = algo_spec(algo);
+    return lds_need_iface(e, sp.iface, sp.xchg_lds);
 }
 
 template <typename T>
@@ -200,8 +203,13 @@ static const TileEntry<T> *choose_tile(int algo, int layout, int64_t n, bool vec
         const int prec = sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE;
         // packed systems are solved as contiguous ones (system stride 4n): same rows
         const int tl = layout == kPacked ? kContiguous : layout;
+        // rows tuned with the algorithm's own interface solver (family = its id) first,
+        // then the default rows (family 0, tuned with Wang-Mou merges)
+        const int fam = algo_spec(algo).iface;
+        const int fams[2] = {fam, 0};
+        for (int fi = 0; fi < (fam == 0 ? 1 : 2); ++fi)
         for (const TuneRow &r : g_tune) {
-            if (r.prec != prec || r.layout != tl || r.family != 0 || n > r.nmax) continue;
+            if (r.prec != prec || r.layout != tl || r.family != fams[fi] || n > r.nmax) continue;
             if (r.vec && !vec_ok) continue;
             const int nws = il ? r.NW : r.NW / r.S;
             const int64_t cap = (int64_t)(il ? 64 / r.S : 64) * nws * r.M;
@@ -563,11 +571,14 @@ static int tile_info_impl(int index, int *M, int *NW, int *S, int *il, int *vec,
 }
 
 template <typename T>
-static int solve_tile_impl(int index, int layout, int64_t n, int64_t batch, T *a, T *b, T *c,
-                           T *d, hipStream_t st) {
+static int solve_tile_impl(int index, int iface, int layout, int64_t n, int64_t batch, T *a,
+                           T *b, T *c, T *d, hipStream_t st) {
     TileTable<T> tab = tile_table<T>();
     if (index < 0 || index >= tab.count) return fail(TRIDIAG_EINVAL, "tile index out of range");
+    if (iface < kIfaceWM || iface > kIfaceCR) return fail(TRIDIAG_EINVAL, "interface solver id out of range");
     const TileEntry<T> &e = tab.entries[index];
+    if (lds_need_iface(e, iface, 0) > max_lds_per_block())
+        return fail(TRIDIAG_EUNSUPPORTED, "the interface solver's LDS does not fit beside this tile");
     if (n < 1 || batch < 0 || layout < 0 || layout > 2) return fail(TRIDIAG_EINVAL, "bad shape");
     if (!a || (layout != kPacked && (!b || !c || !d))) return fail(TRIDIAG_EINVAL, "null pointer");
     if (e.IL != (layout == kInterleaved)) return fail(TRIDIAG_EINVAL, "tile/layout mismatch");
@@ -588,7 +599,7 @@ static int solve_tile_impl(int index, int layout, int64_t n, int64_t batch, T *a
         return fail(TRIDIAG_EINVAL, "vector tile needs 16-byte aligned rows");
     const int32_t bb = buffer_extent<T>(n, batch, sstride, layout);
     if (bb == 0 && !e.BIG) return fail(TRIDIAG_EUNSUPPORTED, "arrays of 2 GiB or more need a BIG tile");
-    TileParams<T> p{a, b, c, d, n, batch, sstride, rstride, bb, kIfaceWM, 0};
+    TileParams<T> p{a, b, c, d, n, batch, sstride, rstride, bb, iface, 0};
     hipError_t err = e.launch(p, st);
     return err == hipSuccess ? TRIDIAG_OK : hip_fail(err, "tile launch");
 }
@@ -970,16 +981,22 @@ int tridiag_tile_info(int precision, int index, int *M, int *NW, int *S, int *in
     return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
 }
 
-int tridiag_solve_tile(int precision, int index, int layout, int64_t n_eqt, int64_t n_batch,
-                       void *a, void *b, void *c, void *d, void *stream) {
+int tridiag_solve_tile_iface(int precision, int index, int iface, int layout, int64_t n_eqt,
+                             int64_t n_batch, void *a, void *b, void *c, void *d, void *stream) {
     g_last_error.clear();
     if (precision == TRIDIAG_SINGLE)
-        return solve_tile_impl<float>(index, layout, n_eqt, n_batch, (float *)a, (float *)b,
+        return solve_tile_impl<float>(index, iface, layout, n_eqt, n_batch, (float *)a, (float *)b,
                                       (float *)c, (float *)d, (hipStream_t)stream);
     if (precision == TRIDIAG_DOUBLE)
-        return solve_tile_impl<double>(index, layout, n_eqt, n_batch, (double *)a, (double *)b,
-                                       (double *)c, (double *)d, (hipStream_t)stream);
+        return solve_tile_impl<double>(index, iface, layout, n_eqt, n_batch, (double *)a,
+                                       (double *)b, (double *)c, (double *)d, (hipStream_t)stream);
     return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
+}
+
+int tridiag_solve_tile(int precision, int index, int layout, int64_t n_eqt, int64_t n_batch,
+                       void *a, void *b, void *c, void *d, void *stream) {
+    return tridiag_solve_tile_iface(precision, index, kIfaceWM, layout, n_eqt, n_batch, a, b, c, d,
+                                    stream);
 }
 
 int64_t tridiag_ragged_work_size(int64_t n_sys) { return n_sys < 0 ? 0 : ragged_work_bytes(n_sys); }
