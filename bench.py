@@ -200,6 +200,30 @@ def run_workload(torch, T, name, dtname, layout, steps, warmup):
     return r
 
 
+def run_pcr(torch, T, steps, warmup, algo=5):
+    """The PCR-pThomas on-chip solver (reference id 5, reg PCR-pThomas) at n=4096 x
+    4096, FP32 and FP64, contiguous and interleaved, timed and checked like the
+    automatic plan."""
+    import numpy as np
+    n, batch = N_EQT, N_BATCH
+    out = {}
+    for name, dtname, layout in WORKLOADS:
+        dtype = np.dtype(dtname).type
+        log("prepare pcr", name)
+        sets = make_sets(torch, dtype, n, batch, N_SETS, seed0=11, layout=layout)
+        wall, kern = time_solver(torch, T, sets, n, batch, steps, warmup, algo, layout)
+        del sets
+        torch.cuda.empty_cache()
+        w = np.dtype(dtype).itemsize
+        par = check_parity(torch, T, dtype, n, batch, algo, layout, 1e-5 if w == 4 else 1e-12)
+        achieved = 5 * w * n * batch / (kern * 1e-3) / 1e9
+        out[name] = dict(kernel_ms=kern, ms_per_step=wall, unknowns_per_s=n * batch / (wall * 1e-3),
+                         hbm_frac=round(achieved / HBM_PEAK_GBS, 4),
+                         plan=T.plan(0 if w == 4 else 1, algo, layout, n, batch), parity=par)
+        log("pcr", name, json.dumps(out[name]))
+    return {"algo": algo, "name": "reg PCR-pThomas", **out}
+
+
 def _event_time(torch, fn, steps, stream):
     """Runs fn() `steps` times between two HIP events on `stream`; returns
     (wall ms per call, event ms per call)."""
@@ -613,6 +637,7 @@ def main():
     for name, dtname, layout in WORKLOADS:
         res[name] = run_workload(torch, T, name, dtname, layout, args.steps, args.warmup)
     main_r = res["f32_contiguous"]
+    pcr = run_pcr(torch, T, args.steps, args.warmup)
     large = run_large(torch, T, args.steps, args.warmup)
     ragged = run_ragged(torch, T, args.steps, args.warmup)
     fwdbwd = run_fwd_bwd(torch, T, args.steps, args.warmup)
@@ -645,6 +670,7 @@ def main():
                                                "reference_style_sync_ms", "layout", "plan",
                                                "parity", "roofline")}
                       for k, v in res.items()},
+        "pcr_4096x4096": pcr,
         "large": large,
         "ragged": ragged,
         "n1024_f64": n1024,
