@@ -62,6 +62,13 @@ def test_argument_validation_without_gpu(capi):
     assert L.tridiag_solve(0, 9, 0, 16, 4, nul, nul, nul, nul, nul) == capi.EINVAL
     assert b"null" in L.tridiag_last_error()
     assert L.tridiag_solve(0, 9, 5, 16, 4, nul, nul, nul, nul, nul) == capi.EINVAL
+    # forced tile solves: interface solver id, precision and tile index are checked
+    args = (0, 16, 4, nul, nul, nul, nul, nul)
+    assert L.tridiag_solve_tile_iface(0, 0, 5, *args) == capi.EINVAL
+    assert L.tridiag_solve_tile_iface(0, 0, -1, *args) == capi.EINVAL
+    assert b"interface" in L.tridiag_last_error()
+    assert L.tridiag_solve_tile_iface(3, 0, 2, *args) == capi.EINVAL
+    assert L.tridiag_solve_tile_iface(0, -1, 2, *args) == capi.EINVAL
     # the vendor comparator (algo 0) needs layout 0 and at least 3 rows
     assert L.tridiag_solve(0, 0, 2, 16, 4, nul, nul, nul, nul, nul) == capi.EUNSUPPORTED
     assert L.tridiag_solve(1, 0, 0, 2, 4, nul, nul, nul, nul, nul) == capi.EUNSUPPORTED

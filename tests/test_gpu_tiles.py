@@ -3,7 +3,8 @@
 Each tile (rows per lane M, waves NW, systems per workgroup S, layout and load
 variant) is run through tridiag_solve_tile at its capacity, and padded tiles also
 at a size that leaves part of the last lane segment empty; the batch leaves the
-last workgroup partly idle.  Tolerance: normwise relative error <= 1e-5 (FP32) /
+last workgroup partly idle.  Every tile also runs under each of the other interface
+solvers (tridiag_solve_tile_iface), or reports that the solver's LDS does not fit.  Tolerance: normwise relative error <= 1e-5 (FP32) /
 <= 1e-12 (FP64) on every system.
 """
 import ctypes
@@ -12,6 +13,7 @@ import numpy as np
 import pytest
 
 from gpu_util import TOL, announce, check_against_oracle, make_case
+from tridiagonalsolver_amd import _capi
 
 pytestmark = pytest.mark.gpu
 
@@ -25,15 +27,20 @@ def _gpu():
         pytest.skip("no GPU")
 
 
-def _run_tile(prec, t, n, batch, seed):
+def _run_tile(prec, t, n, batch, seed, iface=None):
     import torch
     import tridiagonalsolver_amd as T
     layout = 2 if t["interleaved"] else 0
     arrs = make_case(n, batch, layout, DT[prec], seed)
     ts = [torch.from_numpy(np.ascontiguousarray(v)).cuda() for v in arrs]
-    rc = T.lib().tridiag_solve_tile(prec, t["index"], layout, n, batch,
-                                    *(ctypes.c_void_p(x.data_ptr()) for x in ts),
-                                    ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    ptrs = [ctypes.c_void_p(x.data_ptr()) for x in ts]
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if iface is None:
+        rc = T.lib().tridiag_solve_tile(prec, t["index"], layout, n, batch, *ptrs, st)
+    else:
+        rc = T.lib().tridiag_solve_tile_iface(prec, t["index"], iface, layout, n, batch, *ptrs, st)
+        if rc == _capi.EUNSUPPORTED and b"LDS" in T.lib().tridiag_last_error():
+            return None, None
     T.check(rc)
     torch.cuda.synchronize()
     x = ts[3].cpu().numpy()
@@ -59,4 +66,30 @@ def test_every_tile(prec, interleaved):
             st, res = _run_tile(prec, t, n, batch, seed=t["index"] * 7 + n)
             if not (st["nonfinite"] == 0 and st["relerr"] <= TOL[DT[prec]]):
                 fails.append((t, n, st))
+    assert not fails, fails[:5]
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("interleaved", [False, True])
+def test_every_tile_every_interface_solver(prec, interleaved):
+    """Wang-Mou half (1), PCR (2), PCR + Thomas (3) and CR (4) on every tile, at the
+    tile's first test size."""
+    import tridiagonalsolver_amd as T
+    fails, ran = [], 0
+    tiles = [t for t in T.tiles(prec) if t["interleaved"] == interleaved and not t["wide_addr"]]
+    for t in tiles:
+        n = t["cap"] if not t["padded"] else max(1, t["cap"] // 2 + 3)
+        if t["vec"]:
+            v = 4 if prec == 0 else 2
+            n = max(v, n - n % v)
+        batch = 2 * t["S"] + 1
+        for iface in (1, 2, 3, 4):
+            announce("tile iface", prec, t, "iface", iface, "n", n, "batch", batch)
+            st, res = _run_tile(prec, t, n, batch, seed=t["index"] * 5 + iface, iface=iface)
+            if st is None:
+                continue
+            ran += 1
+            if not (st["nonfinite"] == 0 and st["relerr"] <= TOL[DT[prec]]):
+                fails.append((t, iface, n, st))
+    assert ran > 0
     assert not fails, fails[:5]
