@@ -7,13 +7,18 @@ one whose answer matches the FP64 reference solution.
 
     python -m tridiagonalsolver_amd.autotune --ragged --prec 0 1 --table tuning/tuning_gfx950.txt
 
+    python -m tridiagonalsolver_amd.autotune --prec 0 1 --layout 0 --iface 1 2 3 4 \
+        --n 1024 4096 --unknowns 16777216 --table tuning/tuning_gfx950.txt
+
 The table file holds one line per bucket,
     precision layout family n_max M NW S vec lds_alpha nt xp
 (family 0 = the default on-chip solver; nt = nontemporal loads/stores;
 xp = whole-line loads handed to lanes through LDS)
 and is read by libtridiag_amd.so at first use (TRIDIAG_TUNING_FILE overrides its
 location): for a solve of n rows the first bucket with n <= n_max whose kernel
-exists is used.  Tuning at sizes between powers of two (for example 768, 1536,
+exists is used.  family is the interface solver the row was tuned with (--iface:
+0 Wang-Mou, 1 Wang-Mou half, 2 PCR, 3 PCR + Thomas, 4 CR); an algorithm looks at
+the rows of its own interface solver first, then at family 0.  Tuning at sizes between powers of two (for example 768, 1536,
 3072, 6144) gives those sizes buckets of their own, tuned on padded kernels.
 Kernels are timed as back-to-back launches on rotated input sets, the way a
 caller streaming solves sees them.  --ragged writes lines "R precision n_max shape iface": the shape
