@@ -149,10 +149,13 @@ int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_
  *                         autotuned table (tuning_gfx950.txt, lines
  *                         "R precision n_max shape iface"); a shape or iface
  *                         >= 0 forces it for every system.
- *   tridiag_ragged_solve  launches the solve for a planned work buffer (two
- *                         kernels, no host synchronisation, no allocation:
+ *   tridiag_ragged_solve  launches the solve for a planned work buffer (one
+ *                         kernel, no host synchronisation, no allocation:
  *                         capturable into a graph and replayable).  Concurrent
- *                         solves on one work buffer race.
+ *                         solves on one work buffer race.  Returns
+ *                         TRIDIAG_EINVAL when `work` was not planned by
+ *                         tridiag_ragged_plan in this process, or was planned
+ *                         for another precision or system count.
  *   tridiag_solve_ragged  plan + solve.
  *
  * No counterpart in the reference, whose entry points take one n_eqt for the

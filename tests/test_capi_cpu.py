@@ -73,6 +73,10 @@ def test_argument_validation_without_gpu(capi):
     assert L.tridiag_solve(0, 0, 2, 16, 4, nul, nul, nul, nul, nul) == capi.EUNSUPPORTED
     assert L.tridiag_solve(1, 0, 0, 2, 4, nul, nul, nul, nul, nul) == capi.EUNSUPPORTED
     assert b">= 3" in L.tridiag_last_error()
+    # a ragged solve needs a work buffer planned in this process
+    fake = ctypes.c_void_p(0x1000)
+    assert L.tridiag_ragged_solve(0, 4, fake, fake, fake, fake, fake, nul) == capi.EINVAL
+    assert b"no plan" in L.tridiag_last_error()
 
 
 @pytest.mark.parametrize("prec", [0, 1])

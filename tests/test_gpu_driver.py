@@ -12,6 +12,15 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "tridiagonalsolver_amd", "lib", "tridiag_test")
+DRIVER_SRC = os.path.join(ROOT, "tridiagonalsolver_amd", "driver", "tridiag_test.cpp")
+LIBRARY = os.path.join(ROOT, "tridiagonalsolver_amd", "lib", "libtridiag_amd.so")
+
+
+def _driver_stale():
+    """True when the built driver predates its source or the library it runs
+    against (a stale binary would test an old build)."""
+    t = os.path.getmtime(DRIVER)
+    return any(os.path.exists(p) and os.path.getmtime(p) > t for p in (DRIVER_SRC, LIBRARY))
 
 
 @pytest.mark.parametrize("prec,algo,n,batch,tol", [
@@ -21,6 +30,8 @@ DRIVER = os.path.join(ROOT, "tridiagonalsolver_amd", "lib", "tridiag_test")
 def test_driver_run(tmp_path, prec, algo, n, batch, tol):
     if not os.path.exists(DRIVER):
         pytest.skip("driver not built")
+    if _driver_stale():
+        pytest.skip("driver binary is older than its source or the library: rebuild it")
     csv = tmp_path / "res.csv"
     announce("driver", prec, algo, n, batch)
     r = subprocess.run([DRIVER, str(prec), str(algo), str(n), str(batch), "3", str(csv)],

@@ -260,6 +260,29 @@ def test_plan_errors():
         plan.solve(x, x, x, x)
 
 
+def test_solve_rejects_mismatched_plan():
+    """A work buffer planned for one precision or system count is refused by a
+    solve of another through the C-ABI (the kernel alone would skip it silently)."""
+    import torch
+    import tridiagonalsolver_amd as T
+    from tridiagonalsolver_amd import _capi
+    L = _capi.lib()
+    dev = torch.device("cuda:0")
+    off = [0, 10, 30]
+    plan = T.RaggedPlan(off, dtype=torch.float32, device=dev)
+    x32 = torch.ones(30, dtype=torch.float32, device=dev)
+    x64 = torch.ones(30, dtype=torch.float64, device=dev)
+    p = [ctypes.c_void_p(t.data_ptr()) for t in (x64, x64, x64, x64)]
+    w = ctypes.c_void_p(plan.work.data_ptr())
+    assert L.tridiag_ragged_solve(1, 2, *p, w, None) == _capi.EINVAL
+    assert b"single" in L.tridiag_last_error()
+    p = [ctypes.c_void_p(t.data_ptr()) for t in (x32, x32, x32, x32)]
+    assert L.tridiag_ragged_solve(0, 1, *p, w, None) == _capi.EINVAL
+    assert b"2 systems" in L.tridiag_last_error()
+    assert L.tridiag_ragged_solve(0, 2, *p, w, None) == _capi.OK
+    torch.cuda.synchronize()
+
+
 def test_full_size_workload():
     """65,536 systems, n log-uniform in [32, 8192], FP32: every system against the oracle."""
     sizes = oracle.ragged_sizes(65536, 32, 8192, seed=2024)

@@ -213,15 +213,33 @@ def _compile(src):
     return obj, r.stderr
 
 
-def build(verbose=False, jobs=None):
+def _headers_mtime():
+    inc = os.path.join(os.path.dirname(PKG), "include")
+    hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
+    hs += [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith(".h")]
+    return max(os.path.getmtime(h) for h in hs) if hs else 0.0
+
+
+def build(verbose=False, jobs=None, incremental=False):
+    """Compiles every csrc/*.hip and links the library.  incremental=True keeps
+    objects newer than their source and every header (a development shortcut; the
+    default rebuilds everything)."""
     generate()
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
-    jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
-    objs = []
+    hdr_t = _headers_mtime() if incremental else 0.0
+    todo, objs = [], []
+    for src in srcs:
+        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+        if incremental and os.path.exists(obj) and \
+                os.path.getmtime(obj) > max(os.path.getmtime(src), hdr_t):
+            objs.append(obj)
+        else:
+            todo.append(src)
+    jobs = jobs or min(max(1, len(todo)), max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        for obj, log in ex.map(_compile, srcs):
+        for obj, log in ex.map(_compile, todo):
             objs.append(obj)
             if verbose and log.strip():
                 print(log, file=sys.stderr)
@@ -244,4 +262,4 @@ def build(verbose=False, jobs=None):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))

@@ -495,7 +495,6 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
 static hipsparseHandle_t g_sparse = nullptr;
 static void *g_sparse_buf = nullptr;
 static size_t g_sparse_bytes = 0;
-static void *g_cr_buf = nullptr;
 
 template <typename T> static int sparse_init(T *a, T *b, T *c, T *d, int n, int batch) {
     if (!g_sparse) {
@@ -635,6 +634,44 @@ static void ragged_choice(int prec, int64_t n, int &shape, int &iface) {
     iface = kIfaceWM;
 }
 
+// Plans written by ragged_plan_impl, by work-buffer address: the system count
+// and precision each one was made for.  The kernel ignores a work buffer whose
+// header does not match its launch (it then leaves d untouched), so the host
+// checks the same fields here and reports the mismatch instead.
+struct RaggedPlanKey {
+    int64_t n_sys;
+    int prec;
+};
+static std::mutex g_rplan_mu;
+static std::vector<std::pair<const void *, RaggedPlanKey>> g_rplans;
+
+static void ragged_plan_record(const void *work, int64_t n_sys, int prec) {
+    std::lock_guard<std::mutex> lk(g_rplan_mu);
+    for (auto &e : g_rplans)
+        if (e.first == work) {
+            e.second = {n_sys, prec};
+            return;
+        }
+    g_rplans.push_back({work, {n_sys, prec}});
+}
+
+static int ragged_plan_check(const void *work, int64_t n_sys, int prec) {
+    std::lock_guard<std::mutex> lk(g_rplan_mu);
+    for (const auto &e : g_rplans) {
+        if (e.first != work) continue;
+        if (e.second.prec != prec)
+            return fail(TRIDIAG_EINVAL, std::string("the work buffer holds a plan for ") +
+                                            (e.second.prec == TRIDIAG_SINGLE ? "single" : "double") +
+                                            " precision");
+        if (e.second.n_sys != n_sys)
+            return fail(TRIDIAG_EINVAL, "the work buffer holds a plan for " +
+                                            std::to_string(e.second.n_sys) + " systems, not " +
+                                            std::to_string(n_sys));
+        return TRIDIAG_OK;
+    }
+    return fail(TRIDIAG_EINVAL, "the work buffer holds no plan (call tridiag_ragged_plan first)");
+}
+
 static int64_t ragged_work_bytes(int64_t n_sys) {
     const int64_t raw = kRaggedHeaderBytes + (int64_t)kRaggedRecInts * 4 * n_sys;
     return (raw + 255) & ~(int64_t)255;
@@ -726,7 +763,9 @@ static int ragged_plan_impl(int prec, int64_t n_sys, const int64_t *off, int for
     hipError_t e = hipMemcpyAsync(work, host.data(), used, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "ragged plan upload");
     e = hipStreamSynchronize(st);
-    return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "ragged plan upload");
+    if (e != hipSuccess) return hip_fail(e, "ragged plan upload");
+    ragged_plan_record(work, n_sys, prec);
+    return TRIDIAG_OK;
 }
 
 template <typename T>
@@ -735,6 +774,7 @@ static int ragged_solve_impl(int prec, int64_t n_sys, T *a, T *b, T *c, T *d, vo
     if (n_sys < 0 || n_sys > 0x7fffffff) return fail(TRIDIAG_EINVAL, "n_sys out of range");
     if (n_sys == 0) return TRIDIAG_OK;
     if (!work || !a || !b || !c || !d) return fail(TRIDIAG_EINVAL, "null pointer");
+    if (const int rc = ragged_plan_check(work, n_sys, prec)) return rc;
     char *w = reinterpret_cast<char *>(work);
     RaggedParams<T> p{a, b, c, d, reinterpret_cast<RaggedHeader *>(w),
                       reinterpret_cast<const int32_t *>(w + kRaggedHeaderBytes), n_sys, prec};
@@ -1157,10 +1197,9 @@ int CR_init_double(double *, double *, double *, double *, int n_eqt, int n_batc
     (void)n_eqt; (void)n_batch;
     return 0;
 }
-void CR_final(void) {
-    if (g_cr_buf) (void)hipFree(g_cr_buf);
-    g_cr_buf = nullptr;
-}
+// The CR solve keeps no state between calls (its reduction runs on chip), so
+// CR_init and CR_final have nothing to allocate or release.
+void CR_final(void) {}
 
 int cuSparse_init_single(float *a, float *b, float *c, float *d, int n_eqt, int n_batch) {
     return sparse_init<float>(a, b, c, d, n_eqt, n_batch);
