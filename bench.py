@@ -107,27 +107,122 @@ def check_parity(torch, T, dtype, n, batch, algo, layout, tol):
                 ok=bool(st["relerr"] <= tol))
 
 
-def cpu_baseline(budget_s=12.0):
-    """The oracle's single-precision port of the reference CPU Thomas
-    (Tridiagonal_AVX512/tridiagonal_basic.cpp:6-26), OpenMP over the host cores,
-    timed on a bounded sample of the same workload (layout 0, n=4096)."""
+def _l3_bytes():
+    """Total last-level cache of the host: the sum over distinct L3 instances."""
+    seen, total = set(), 0
+    base = "/sys/devices/system/cpu"
+    try:
+        cpus = [c for c in os.listdir(base) if c.startswith("cpu") and c[3:].isdigit()]
+    except OSError:
+        cpus = []
+    for cpu in cpus:
+        cdir = os.path.join(base, cpu, "cache")
+        try:
+            idx = os.listdir(cdir)
+        except OSError:
+            continue
+        for ix in idx:
+            try:
+                with open(os.path.join(cdir, ix, "level")) as f:
+                    if f.read().strip() != "3":
+                        continue
+                with open(os.path.join(cdir, ix, "shared_cpu_list")) as f:
+                    key = f.read().strip()
+                with open(os.path.join(cdir, ix, "size")) as f:
+                    sz = f.read().strip()
+            except OSError:
+                continue
+            if key in seen:
+                continue
+            seen.add(key)
+            mult = {"K": 1 << 10, "M": 1 << 20, "G": 1 << 30}.get(sz[-1:], 1)
+            total += int(sz.rstrip("KMG")) * mult
+    return total or (256 << 20)
+
+
+# Points of the CPU baseline: the reference's CPU solvers (restated, oracle/cpu_ref.cpp)
+# at the reference's CPU-runnable case and at the headline shape, layout 0
+# (the reference CPU kernels have no interleaved layout).  (name, dtype, n, batch, algo)
+CPU_POINTS = [
+    ("f64_n1024_b1024_thomas", "float64", 1024, 1024, 1),
+    ("f64_n1024_b1024_wm_pge", "float64", 1024, 1024, 5),
+    ("f32_n4096_b4096_pcr_pthomas", "float32", 4096, 4096, 3),
+    ("f32_n4096_b4096_wm_pge", "float32", 4096, 4096, 5),
+    ("f64_n4096_b4096_pcr_pthomas", "float64", 4096, 4096, 3),
+    ("f64_n4096_b4096_wm_pge", "float64", 4096, 4096, 5),
+]
+CPU_HEADLINE = "f32_n4096_b4096_wm_pge"
+
+
+def cpu_leg():
+    """Runs in a child process (so OpenMP reads the placement below at start-up):
+    the restated reference CPU solvers over every host CPU this process may use,
+    threads spread over the cores.  Each point solves copies of one seeded batch;
+    the copies together exceed twice the host's L3, and each timed solve starts
+    from a copy the others have since pushed out of cache.  Parity of every point's
+    last solve against the FP64 oracle is reported with it."""
     import numpy as np
     import oracle
-    nthreads = _host_threads()
-    n, batch = N_EQT, 512
-    a, b, c, d = (v.astype(np.float32) for v in oracle.gen_dd(n, batch, layout=0, seed=7))
-    oracle.thomas_f32(a, b, c, d, n, batch, 0, nthreads)      # warm
-    reps, t = 0, 0.0
-    while t < budget_s:
-        t0 = time.perf_counter()
-        oracle.thomas_f32(a, b, c, d, n, batch, 0, nthreads)
-        t += time.perf_counter() - t0
-        reps += 1
-    return {"value": n * batch * reps / t, "unit": "unknowns/s", "cores": nthreads,
-            "kind": "port",
-            "sample": "FP32 Thomas (oracle port of tridiagonal_basic.cpp:6-26), n=4096 x "
-                      "batch=512 layout 0, %d reps in %.1f s, OpenMP %d threads" % (reps, t,
-                                                                                    nthreads)}
+    from oracle import cpu_ref
+    info = cpu_ref.cpu_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", info["usable_cpus"]))
+    l3 = _l3_bytes()
+    points = {}
+    t_start = time.perf_counter()
+    gen_cache = {}
+    for name, dtname, n, batch, algo in CPU_POINTS:
+        dtype = np.dtype(dtname).type
+        key = (n, batch)
+        if key not in gen_cache:
+            gen_cache.clear()
+            gen_cache[key] = oracle.gen_dd(n, batch, layout=0, seed=7)
+        a, b, c, d = (v.astype(dtype) for v in gen_cache[key])
+        set_bytes = 4 * n * batch * np.dtype(dtype).itemsize
+        nsets = int(min(64, 1 + -(-2 * l3 // set_bytes)))
+        # one round to size the run, then enough rounds for ~1.5 s of timed solves
+        t1, _, _ = cpu_ref.bench(a, b, c, d, n, batch, algo, nsets, 1, threads)
+        rounds = int(max(1, min(20, 1.5 / (t1 * nsets))))
+        t, tmin, x = cpu_ref.bench(a, b, c, d, n, batch, algo, nsets, rounds, threads)
+        xref = oracle.thomas(*gen_cache[key], n, batch, layout=0) if dtype == np.float64 else \
+            oracle.thomas(*(v.astype(np.float64) for v in (a, b, c, d)), n, batch, layout=0)
+        err = float(oracle.errstats(x.astype(np.float64), xref, n, batch, 0)["relerr"])
+        tol = 1e-5 if dtype == np.float32 else 1e-12
+        points[name] = {"algo": algo, "name": cpu_ref.NAMES[algo], "n": n, "batch": batch,
+                        "dtype": "f32" if dtype == np.float32 else "f64",
+                        "unknowns_per_s": n * batch / t, "ms_per_solve": t * 1e3,
+                        "fastest_ms": tmin * 1e3, "solves": nsets * rounds, "sets": nsets,
+                        "relerr": err, "ok": bool(err <= tol)}
+        del a, b, c, d, x, xref
+    head = points[CPU_HEADLINE]
+    return {
+        "value": head["unknowns_per_s"], "unit": "unknowns/s", "cores": threads,
+        "kind": "port",
+        "algorithm": "reference CPU WM-pGE (Tridiagonal_AVX512 WM4_single), restated",
+        "sample": "FP32 WM-pGE n=4096 x batch=4096 layout 0, %d solves of %d input copies "
+                  "(%.0f MB each, together > 2x the %.0f MB host L3), OpenMP %d threads "
+                  "spread over cores, %s vectors" % (head["solves"], head["sets"],
+                                                       4 * 4 * 4096 * 4096 / 1e6, l3 / 1e6,
+                                                       threads, cpu_ref.simd()),
+        "cpu": {"model": info["model"], "usable_cpus": info["usable_cpus"],
+                "avx512f": info["avx512f"], "simd_build": cpu_ref.simd(), "l3_bytes": l3},
+        "points": points,
+        "leg_seconds": time.perf_counter() - t_start,
+    }
+
+
+def cpu_baseline():
+    """The CPU leg in a child process with OpenMP placement set before start-up."""
+    import subprocess
+    env = dict(os.environ)
+    env.setdefault("OMP_NUM_THREADS", str(_host_threads()))
+    env["OMP_PROC_BIND"] = "spread"
+    env["OMP_PLACES"] = "cores"
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-only"], env=env,
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        log("cpu leg failed:", r.stderr[-2000:])
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def load_traffic(name):
@@ -504,10 +599,10 @@ def run_n1024(torch, T, steps, warmup, with_cpu, budget_s=2.0):
     not graded.  Besides the event-timed solve, the reference-style figure: the
     synchronous Part-1 entry (WM5_reg_double, Thomas_double) timed by the host
     clock around each call with inputs restored untimed before it, as the reference
-    driver does (Tridiagonal_CUDA/test.cpp:94-100).  With the CPU leg, the FP64
-    oracle Thomas and LAPACK dgtsv (scipy; partial pivoting, the reference's CPU
-    comparator LAPACKE_?gtsv at Tridiagonal_AVX512/tridiagonal_basic.cpp:131-143)
-    on the same workload, reported only."""
+    driver does (Tridiagonal_CUDA/test.cpp:94-100).  With the CPU leg, LAPACK dgtsv
+    (scipy; partial pivoting, the reference's CPU comparator LAPACKE_?gtsv at
+    Tridiagonal_AVX512/tridiagonal_basic.cpp:131-143) on the same workload, reported
+    only; the reference's own CPU solvers at this shape are in cpu_baseline.points."""
     import ctypes
     import numpy as np
     import oracle
@@ -544,17 +639,8 @@ def run_n1024(torch, T, steps, warmup, with_cpu, budget_s=2.0):
     del sets, work
     torch.cuda.empty_cache()
     if with_cpu:
-        nthreads = _host_threads()
         a, b, c, d = oracle.gen_dd(n, batch, layout=0, seed=21)
-        x_or = oracle.thomas(a, b, c, d, n, batch, 0, nthreads)
-        reps, t = 0, 0.0
-        while t < budget_s:
-            t0 = time.perf_counter()
-            oracle.thomas(a, b, c, d, n, batch, 0, nthreads)
-            t += time.perf_counter() - t0
-            reps += 1
-        r["cpu_oracle_thomas"] = {"unknowns_per_s": U * reps / t, "threads": nthreads,
-                                  "reps": reps}
+        x_or = oracle.thomas(a, b, c, d, n, batch, 0)
         try:
             from scipy.linalg import lapack
         except ImportError:
@@ -623,7 +709,12 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="only the CPU baseline leg (no GPU); prints its JSON object")
     args = ap.parse_args()
+    if args.cpu_only:
+        print(json.dumps(cpu_leg()), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (systems are "
               "independent; sharding the batch across GPUs needs no communication).",
