@@ -225,12 +225,22 @@ def cpu_baseline():
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def load_traffic(name):
+def plan_key(plan):
+    """Short name of a solve plan (tridiagonalsolver_amd.plan): kind and shape."""
+    if "M" in plan:
+        return "%s M%d NW%d S%d" % (plan["kind"], plan["M"], plan["NW"], plan["S"])
+    return plan["kind"]
+
+
+def load_traffic(key):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE) for the
+    workload and kernel plan in `key` ("<workload>|<plan>"), from
+    profiles/traffic.json; None when that plan was not profiled."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get(name)
+        return json.load(f).get(key)
 
 
 WORKLOADS = [
@@ -290,7 +300,7 @@ def run_workload(torch, T, name, dtname, layout, steps, warmup):
              layout={0: "contiguous", 2: "interleaved"}[layout],
              roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                       "traffic": load_traffic(name + "_4096x4096")})
+                       "traffic": load_traffic("%s_4096x4096|%s" % (name, plan_key(pl)))})
     log(name, json.dumps(r))
     return r
 
@@ -422,10 +432,17 @@ def run_ragged(torch, T, steps, warmup, n_sys=65536, nmin=32, nmax=8192):
     for _ in range(min(warmup, 2)):
         gp.replay()
     wall_pg, ev_pg = _event_time(torch, gp.replay, psteps, stream)
-    del g, gp, work, src
+    del g, gp, work
+    torch.cuda.empty_cache()
+
+    log("plain bucketed timing")
+    bucketed = _ragged_bucketed(torch, T, arrs, sizes, off, steps, warmup, stream)
+    del src
     torch.cuda.empty_cache()
 
     achieved = alg_bytes / (ev_r * 1e-3) / 1e9
+    ratio = load_traffic("ragged_f32|%d shapes|traffic_per_alg_byte" %
+                         T.lib().tridiag_ragged_shape_count())
     r = {
         "systems": n_sys, "unknowns": U, "distinct_n": int(len(uniq)), "dtype": "f32",
         "sizes": "n = round(exp(U[ln %d, ln %d])), seed 2024, ordered by size" % (nmin, nmax),
@@ -437,14 +454,99 @@ def run_ragged(torch, T, steps, warmup, n_sys=65536, nmin=32, nmax=8192):
                            "unknowns_per_s": U / (wall_p * 1e-3), "relerr": err_plain},
         "plain_per_size_graph": {"ms_per_step": wall_pg, "gpu_ms": ev_pg,
                                  "unknowns_per_s": U / (wall_pg * 1e-3)},
+        "plain_bucketed": bucketed,
         "speedup_vs_plain_graph": wall_pg / wall_rg,
+        "speedup_vs_bucketed_graph": bucketed["graph"]["ms_per_step"] / wall_rg,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": (round(load_traffic("ragged_f32_traffic_per_alg_byte") * alg_bytes)
-                                 if load_traffic("ragged_f32_traffic_per_alg_byte") else None)},
+                     "traffic": (round(ratio * alg_bytes) if ratio else None)},
     }
     log("ragged", json.dumps(r))
     return r
+
+
+# Capacities of the padded buckets a caller without a ragged solver would use:
+# powers of two and 1.5x powers of two from 32 to 8192 rows.
+RAGGED_BUCKETS = sorted({c for k in range(9) for c in (32 << k, 48 << k) if c <= 8192})
+
+
+def _ragged_bucketed(torch, T, arrs, sizes, off, steps, warmup, stream):
+    """The baseline a caller would use without a ragged solver: systems grouped by
+    size into RAGGED_BUCKETS, each padded with identity rows (a=0, b=1, c=0, d=0)
+    to its bucket's capacity and laid out contiguously, then one plain
+    tridiag_solve per bucket.  Timed eager and graph-replayed like the persistent
+    launch; unknowns/s counts the real (unpadded) unknowns.  The padding is the
+    caller's layout and happens before timing."""
+    import ctypes
+    import numpy as np
+    import oracle
+    U = int(off[-1])
+    dev = torch.device("cuda:0")
+    padded, meta = [], []
+    prev = 0
+    for cap in RAGGED_BUCKETS:
+        lo, hi = np.searchsorted(sizes, prev, side="right"), np.searchsorted(sizes, cap,
+                                                                             side="right")
+        prev = cap
+        if hi <= lo:
+            continue
+        ns = sizes[lo:hi]
+        rows = np.arange(cap)
+        mask = rows[None, :] < ns[:, None]
+        srcix = np.where(mask, off[lo:hi][:, None] + rows[None, :], 0)
+        fill = (0.0, 1.0, 0.0, 0.0)
+        padded.append([torch.from_numpy(np.where(mask, v[srcix], f).astype(np.float32).ravel())
+                       .to(dev) for v, f in zip(arrs, fill)])
+        meta.append((cap, hi - lo, lo, hi, mask))
+        del srcix
+    L = T.lib()
+
+    def args_for(handle):
+        return [(0, -1, 0, cap, cnt) + tuple(ctypes.c_void_p(t.data_ptr()) for t in pb) + (handle,)
+                for (cap, cnt, _, _, _), pb in zip(meta, padded)]
+    bargs = args_for(ctypes.c_void_p(stream.cuda_stream))
+
+    def sweep(a=bargs):
+        rc = 0
+        for x in a:
+            rc |= L.tridiag_solve(*x)
+        return rc
+    # parity: the real rows of every bucket against the FP64 oracle
+    keep = [t[3].clone() for t in padded]
+    T.check(sweep())
+    torch.cuda.synchronize()
+    x = np.empty(U)
+    for (cap, cnt, lo, hi, mask), pb in zip(meta, padded):
+        xb = pb[3].cpu().numpy().reshape(cnt, cap)
+        x[off[lo]:off[hi]] = xb[mask]
+    for pb, k in zip(padded, keep):
+        pb[3].copy_(k)
+    del keep
+    xref = oracle.thomas_ragged(*arrs, off)
+    err = _ragged_relerr(x, xref, off)
+    del xref, x
+    for _ in range(warmup):
+        sweep()
+    _settle(torch, sweep)
+    wall, ev = _event_time(torch, sweep, steps, stream)
+    s = torch.cuda.Stream()
+    s.wait_stream(stream)
+    g = torch.cuda.CUDAGraph()
+    sargs = args_for(ctypes.c_void_p(s.cuda_stream))
+    with torch.cuda.graph(g, stream=s):
+        T.check(sweep(sargs))
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        g.replay()
+    _settle(torch, g.replay)
+    wall_g, ev_g = _event_time(torch, g.replay, steps, stream)
+    padded_rows = int(sum(cap * cnt for cap, cnt, _, _, _ in meta))
+    del g, padded
+    return {"launches": len(meta), "buckets": [m[0] for m in meta],
+            "padded_unknowns": padded_rows, "relerr": err,
+            "eager": {"ms_per_step": wall, "gpu_ms": ev, "unknowns_per_s": U / (wall * 1e-3)},
+            "graph": {"ms_per_step": wall_g, "gpu_ms": ev_g,
+                      "unknowns_per_s": U / (wall_g * 1e-3)}}
 
 
 def run_vendor(torch, T, steps, warmup, ours):
@@ -515,63 +617,77 @@ def run_convert(torch, T, steps, warmup, n=N_EQT, batch=N_BATCH):
     return out
 
 
-def run_fwd_bwd(torch, T, steps, warmup, n=512, batch=8192, layout=0):
+def run_fwd_bwd(torch, T, steps, warmup, n=512, batch=8192, layout=0, min_bytes=512 << 20):
     """Differentiable solve, FP32, n x batch: plain forward solve vs the fused
-    forward + adjoint kernel (and the stand-alone adjoint), inputs resident."""
+    forward + adjoint kernel (and the stand-alone adjoint).  Input sets (a, b, c,
+    d, g and each call's outputs) are rotated, at least min_bytes of inputs in
+    total, so that no call reads what an earlier one left in the 256 MB MALL: the
+    ratio is between HBM-bound figures.  The forward solve runs in place on its
+    set's own copy of d (the values drift from call to call; the work does not)."""
     import ctypes
     import numpy as np
     import oracle
     log("prepare fwd/bwd")
     dev = torch.device("cuda:0")
-    a, b, c, d = oracle.gen_dd(n, batch, layout=layout, seed=41)
-    w = np.random.default_rng(42).standard_normal(n * batch)          # L = sum(w * x)
-    arrs = [v.astype(np.float32) for v in (a, b, c, d)]
-    ts = [torch.from_numpy(v).to(dev) for v in arrs]
-    g = torch.from_numpy(w.astype(np.float32)).to(dev)
-    outs = [torch.empty_like(ts[0]) for _ in range(5)]
-    xf = ts[3].clone()
+    U = n * batch
+    nsets = max(3, -(-min_bytes // (5 * 4 * U)))
+    w = np.random.default_rng(42).standard_normal(U)          # L = sum(w * x)
+    sets = []
+    for k in range(nsets):
+        a, b, c, d = oracle.gen_dd(n, batch, layout=layout, seed=41 + k)
+        ins = [torch.from_numpy(v.astype(np.float32)).to(dev) for v in (a, b, c, d)]
+        g = torch.from_numpy(np.roll(w, 977 * k).astype(np.float32)).to(dev)
+        outs = [torch.empty_like(g) for _ in range(5)]          # x, ga, gb, gc, gd
+        adj = [torch.empty_like(g) for _ in range(4)]
+        sets.append({"in": ins, "g": g, "out": outs, "adj": adj, "xf": ins[3].clone()})
     stream = torch.cuda.current_stream()
     h = ctypes.c_void_p(stream.cuda_stream)
     L = T.lib()
     P = lambda t: ctypes.c_void_p(t.data_ptr())
-    fwd_args = (0, -1, layout, n, batch, P(ts[0]), P(ts[1]), P(ts[2]), P(xf), h)
-    fb_args = (0, layout, n, batch, P(ts[0]), P(ts[1]), P(ts[2]), P(ts[3]), P(g)) + \
-        tuple(P(o) for o in outs) + (h,)
-    adj_outs = [torch.empty_like(g) for _ in range(4)]
-    adj_keep = (0, layout, n, batch, P(ts[0]), P(ts[1]), P(ts[2]), P(outs[0]), P(g)) + \
-        tuple(P(o) for o in adj_outs) + (h,)
-    T.check(L.tridiag_solve_fwd_bwd(*fb_args))
+    fwd_args = [(0, -1, layout, n, batch, P(z["in"][0]), P(z["in"][1]), P(z["in"][2]),
+                 P(z["xf"]), h) for z in sets]
+    fb_args = [(0, layout, n, batch) + tuple(P(t) for t in z["in"]) + (P(z["g"]),) +
+               tuple(P(o) for o in z["out"]) + (h,) for z in sets]
+    adj_args = [(0, layout, n, batch, P(z["in"][0]), P(z["in"][1]), P(z["in"][2]),
+                 P(z["out"][0]), P(z["g"])) + tuple(P(o) for o in z["adj"]) + (h,)
+                for z in sets]
+    for a_ in fb_args:
+        T.check(L.tridiag_solve_fwd_bwd(*a_))
     torch.cuda.synchronize()
-    # parity of the fused outputs against the FP64 oracle
-    a64, b64, c64, d64 = (v.astype(np.float64) for v in arrs)
-    xr = oracle.thomas(a64, b64, c64, d64, n, batch, layout=layout)
-    gr = oracle.adjoint(a64, b64, c64, xr, w.astype(np.float32).astype(np.float64), n, batch,
+    # parity of set 0's fused outputs against the FP64 oracle
+    arrs = [t.cpu().numpy().astype(np.float64) for t in sets[0]["in"]]
+    xr = oracle.thomas(*arrs, n, batch, layout=layout)
+    gr = oracle.adjoint(*arrs[:3], xr, sets[0]["g"].cpu().numpy().astype(np.float64), n, batch,
                         layout=layout)
     errs = [float(oracle.errstats(o.cpu().numpy().astype(np.float64), r, n, batch,
-                                  layout)["relerr"]) for o, r in zip(outs, (xr,) + tuple(gr))]
-    for _ in range(warmup):
-        L.tridiag_solve(*fwd_args)
-        L.tridiag_solve_fwd_bwd(*fb_args)
-        L.tridiag_adjoint(*adj_keep)
-    _settle(torch, lambda: L.tridiag_solve_fwd_bwd(*fb_args))
+                                  layout)["relerr"]) for o, r in zip(sets[0]["out"], (xr,) + tuple(gr))]
+    del arrs, xr, gr
     rc = [0]
+    turn = [0]
 
-    def f1():
-        rc[0] |= L.tridiag_solve(*fwd_args)
-
-    def f2():
-        rc[0] |= L.tridiag_solve_fwd_bwd(*fb_args)
-
-    def f3():
-        rc[0] |= L.tridiag_adjoint(*adj_keep)
+    def rot(fn, args):
+        def f():
+            turn[0] += 1
+            rc[0] |= fn(*args[turn[0] % nsets])
+        return f
+    f1 = rot(L.tridiag_solve, fwd_args)
+    f2 = rot(L.tridiag_solve_fwd_bwd, fb_args)
+    f3 = rot(L.tridiag_adjoint, adj_args)
+    for _ in range(warmup):
+        f1(); f2(); f3()
+    _settle(torch, f2)
     wf, ef = _event_time(torch, f1, steps, stream)
+    _settle(torch, f2)
     wb, eb = _event_time(torch, f2, steps, stream)
+    _settle(torch, f3)
     wa, ea = _event_time(torch, f3, steps, stream)
     T.check(rc[0])
-    U = n * batch
+    del sets
+    torch.cuda.empty_cache()
     r = {
         "n": n, "batch": batch, "layout": {0: "contiguous", 2: "interleaved"}[layout],
-        "dtype": "f32",
+        "dtype": "f32", "input_sets": nsets,
+        "input_bytes_rotated": nsets * 5 * 4 * U,
         "forward": {"kernel_ms": ef, "unknowns_per_s": U / (ef * 1e-3),
                     "gbs": 5 * 4 * U / (ef * 1e-3) / 1e9},
         "fused_fwd_bwd": {"kernel_ms": eb, "unknowns_per_s": U / (eb * 1e-3),

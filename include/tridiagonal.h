@@ -24,6 +24,7 @@
 #ifndef TRIDIAGONAL_AMD_H_INCLUDED
 #define TRIDIAGONAL_AMD_H_INCLUDED 1
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -209,9 +210,20 @@ int tridiag_convert_layout(int precision, int src_layout, int dst_layout, int64_
 /* Message of the last failed call on this thread ("" if none). */
 const char *tridiag_last_error(void);
 
-/* Kernel configuration the dispatcher would use (for reports and tuning):
- * fills rows-per-lane M, waves per workgroup NW, systems per workgroup S.
- * Returns TRIDIAG_OK or TRIDIAG_EUNSUPPORTED. */
+/* Kernel configuration the dispatcher would use for arrays with 16-byte
+ * aligned rows (for reports and tuning).
+ *   tridiag_plan_kind  how the batch is solved: 0 one system per lane (short
+ *                      systems), 1 the Thomas recurrence (algorithm 1), 2 one
+ *                      workgroup per system or per S systems, on chip, 3 K
+ *                      workgroups per group of S systems, each holding a block of
+ *                      64/S*NW*M rows (interleaved) or 64*NW*M rows (contiguous),
+ *                      on chip, 4 three passes over chunks through global
+ *                      memory, 5 interleaved systems solved through a contiguous
+ *                      copy; -1 on a bad argument.
+ *   tridiag_plan       for kinds 2 and 3, rows-per-lane M, waves per workgroup
+ *                      NW, systems per workgroup S; TRIDIAG_EUNSUPPORTED for the
+ *                      other kinds. */
+int tridiag_plan_kind(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch);
 int tridiag_plan(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch, int *M,
                  int *NW, int *S);
 

@@ -103,14 +103,24 @@ def test_chunked_below_onchip_capacity(prec, frac, algo):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_plan_reports_chunked_sizes(prec):
+def test_plan_reports_sizes(prec):
+    import ctypes
     import tridiagonalsolver_amd as T
+    from tridiagonalsolver_amd import _capi
     lo = 2 * CHUNK[prec]
-    assert T.plan(prec, -1, 0, lo, 64)["M"] > 0          # two chunks' worth: on-chip tile
-    with pytest.raises(T.TridiagError) as e:
-        T.plan(prec, -1, 0, lo + 1, 64)                   # above: the chunked solve
-    assert e.value.code == 2
-    assert T.plan(prec, -1, 2, 2048, 64)["M"] > 0          # interleaved: tiles only
+    assert T.plan(prec, -1, 0, lo, 64)["kind"] == "tile"   # two chunks' worth: on-chip tile
+    # above: K workgroups per system (Wang-Mou family) or the chunked solve (others)
+    assert T.plan(prec, -1, 0, lo + 1, 64)["kind"] == "cooperative"
+    assert T.plan(prec, 5, 0, lo + 1, 64)["kind"] == "chunked"
+    i = ctypes.c_int()
+    assert _capi.lib().tridiag_plan(prec, 5, 0, lo + 1, 64, ctypes.byref(i), ctypes.byref(i),
+                                    ctypes.byref(i)) == _capi.EUNSUPPORTED
+    assert T.plan(prec, -1, 2, 512, 64)["kind"] == "tile"
+    p = T.plan(prec, -1, 2, 4096, 64)
+    assert p["kind"] == "cooperative" and p["K"] * p["block_rows"] == 4096
+    assert T.plan(prec, -1, 2, 5000, 64)["kind"] == "via_contiguous"
+    assert T.plan(prec, 1, 0, 1000, 64)["kind"] == "thomas"
+    assert T.plan(prec, -1, 0, 16, 64)["kind"] == "lane"
 
 
 @pytest.mark.parametrize("prec", [0, 1])

@@ -30,7 +30,7 @@ EXTENDED_SYMBOLS = [
     "tridiag_algo_count", "tridiag_algo_name", "tridiag_algo_layout", "tridiag_solve",
     "tridiag_ragged_work_size", "tridiag_solve_ragged", "tridiag_adjoint",
     "tridiag_solve_fwd_bwd", "tridiag_last_error", "tridiag_plan", "tridiag_tile_count",
-    "tridiag_tile_info", "tridiag_solve_tile", "tridiag_solve_tile_iface",
+    "tridiag_tile_info", "tridiag_solve_tile", "tridiag_solve_tile_iface", "tridiag_plan_kind",
     "tridiag_ragged_shape_count",
     "tridiag_ragged_shape_info", "tridiag_ragged_bucket", "tridiag_ragged_plan",
     "tridiag_ragged_solve", "tridiag_convert_layout",
@@ -76,6 +76,8 @@ def lib():
     L.tridiag_plan.argtypes = [ci, ci, ci, i64, i64, ctypes.POINTER(ci), ctypes.POINTER(ci),
                                ctypes.POINTER(ci)]
     L.tridiag_plan.restype = ci
+    L.tridiag_plan_kind.argtypes = [ci, ci, ci, i64, i64]
+    L.tridiag_plan_kind.restype = ci
     L.tridiag_tile_count.argtypes = [ci]
     L.tridiag_tile_count.restype = ci
     pi = ctypes.POINTER(ci)
@@ -114,11 +116,27 @@ def algo_names(precision):
             for i in range(L.tridiag_algo_count(precision))]
 
 
+PLAN_KINDS = ["lane", "thomas", "tile", "cooperative", "chunked", "via_contiguous"]
+
+
 def plan(precision, algo, layout, n, batch):
-    M, NW, S = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    check(lib().tridiag_plan(precision, algo, layout, n, batch, ctypes.byref(M), ctypes.byref(NW),
-                             ctypes.byref(S)))
-    return dict(M=M.value, NW=NW.value, S=S.value)
+    """How tridiag_solve solves the batch: kind (PLAN_KINDS) and, for on-chip
+    workgroup plans, rows per lane M, waves per workgroup NW, systems per workgroup S
+    (cooperative plans: also the rows per block and blocks per system K)."""
+    L = lib()
+    k = L.tridiag_plan_kind(precision, algo, layout, n, batch)
+    if k < 0:
+        raise TridiagError(EINVAL, L.tridiag_last_error().decode())
+    out = dict(kind=PLAN_KINDS[k])
+    if PLAN_KINDS[k] in ("tile", "cooperative"):
+        M, NW, S = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(L.tridiag_plan(precision, algo, layout, n, batch, ctypes.byref(M),
+                             ctypes.byref(NW), ctypes.byref(S)))
+        out.update(M=M.value, NW=NW.value, S=S.value)
+        if PLAN_KINDS[k] == "cooperative":
+            rows = (64 // S.value if layout == INTERLEAVED else 64) * NW.value * M.value
+            out.update(block_rows=rows, K=-(-n // rows))
+    return out
 
 
 def tiles(precision):

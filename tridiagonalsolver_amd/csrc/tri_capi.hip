@@ -16,6 +16,7 @@
 
 #include "../../include/tridiagonal.h"
 #include "tri_adjoint.hpp"
+#include "tri_coop.hpp"
 #include "tri_host.hpp"
 #include "tri_large.hpp"
 #include "tri_ragged.hpp"
@@ -309,11 +310,19 @@ template <typename T> static int64_t large_min_rows() {
     return v;
 }
 
+// A fixed device buffer that a solve uses in place of stream-ordered
+// allocations (null: allocate).
+struct FixedScratch {
+    void *ptr;
+    size_t bytes;
+};
+
 // Systems above the on-chip capacity: the three-pass chunked solve, in groups
-// of systems, with stream-ordered scratch for the chunk outer forms.
+// of systems, with scratch for the chunk outer forms: stream-ordered, or the
+// fixed buffer `fx` (groups then shrink until their scratch fits it).
 template <typename T>
 static int large_impl(int algo, int64_t n, int64_t batch, int64_t sstride, T *a, T *b, T *c, T *d,
-                      hipStream_t st) {
+                      hipStream_t st, const FixedScratch *fx = nullptr) {
     const int prec = sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE;
     const int shape = 0;
     const int64_t C = large_chunk_rows(prec, shape);
@@ -322,12 +331,23 @@ static int large_impl(int algo, int64_t n, int64_t batch, int64_t sstride, T *a,
         return fail(TRIDIAG_EUNSUPPORTED, "n_eqt=" + std::to_string(n) + " needs more than 1024 "
                                           "chunks of " + std::to_string(C) + " rows");
     const int64_t per_sys = 4 * n * (int64_t)sizeof(T);
-    const int64_t G = std::max<int64_t>(1, std::min<int64_t>({batch, 65535,
-                                                              large_group_bytes() / per_sys}));
+    int64_t G = std::max<int64_t>(1, std::min<int64_t>({batch, 65535,
+                                                        large_group_bytes() / per_sys}));
+    const int64_t sys_scratch = K * 8 * (int64_t)sizeof(T);
     void *scratch = nullptr;
-    const size_t bytes = (size_t)(G * K * 8) * sizeof(T);
-    hipError_t e = hipMallocAsync(&scratch, bytes, st);
-    if (e != hipSuccess) return hip_fail(e, "chunked solve scratch");
+    hipError_t e = hipSuccess;
+    if (fx) {
+        if ((int64_t)fx->bytes < sys_scratch)
+            return fail(TRIDIAG_EUNSUPPORTED, "n_eqt=" + std::to_string(n) + " needs " +
+                                                  std::to_string(sys_scratch) + " bytes of scratch "
+                                                  "per system, more than the fixed " +
+                                                  std::to_string(fx->bytes));
+        G = std::min<int64_t>(G, (int64_t)fx->bytes / sys_scratch);
+        scratch = fx->ptr;
+    } else {
+        e = hipMallocAsync(&scratch, (size_t)(G * sys_scratch), st);
+        if (e != hipSuccess) return hip_fail(e, "chunked solve scratch");
+    }
     const AlgoSpec sp = algo_spec(algo);
     int rc = TRIDIAG_OK;
     for (int64_t g0 = 0; g0 < batch && rc == TRIDIAG_OK; g0 += G) {
@@ -344,8 +364,145 @@ static int large_impl(int algo, int64_t n, int64_t batch, int64_t sstride, T *a,
         else if (e != hipSuccess)
             rc = hip_fail(e, "chunked solve launch");
     }
-    hipError_t ef = hipFreeAsync(scratch, st);
-    if (rc == TRIDIAG_OK && ef != hipSuccess) rc = hip_fail(ef, "chunked solve scratch");
+    if (!fx) {
+        hipError_t ef = hipFreeAsync(scratch, st);
+        if (rc == TRIDIAG_OK && ef != hipSuccess) rc = hip_fail(ef, "chunked solve scratch");
+    }
+    return rc;
+}
+
+// ---------------------------------------------------------------- cooperative solve
+template <typename T> static CoopTable<T> coop_table();
+template <> CoopTable<float> coop_table<float>() { return coop_table_f32(); }
+template <> CoopTable<double> coop_table<double>() { return coop_table_f64(); }
+
+static int64_t env_int(const char *name, int64_t dflt) {
+    const char *e = getenv(name);
+    return e ? (int64_t)atoll(e) : dflt;
+}
+
+// Interleaved systems of at least this many rows take the cooperative solve
+// ($TRIDIAG_COOP_MIN_ROWS overrides for both precisions; a negative value turns
+// the cooperative solve off for both layouts).  Measured on MI355X at 2^24
+// unknowns against the one-workgroup tiles: FP32 1024 rows 0.104 vs 0.093 ms,
+// 2048 rows 0.108 vs 0.122 ms, 4096 rows 0.117 vs 0.118 ms; FP64 2048 rows 0.215
+// vs 0.207 ms, 4096 rows 0.232 vs 0.253 ms.  Above 4096 rows (no interleaved
+// tile) it replaces the solve through a contiguous copy: 8192 rows 0.138 vs
+// 0.201 ms (FP32), 0.257 vs 0.406 ms (FP64).
+static int64_t coop_min_rows_il(int prec) {
+    static const int64_t env = env_int("TRIDIAG_COOP_MIN_ROWS", 0);
+    if (env != 0) return env;
+    return prec == TRIDIAG_SINGLE ? 2048 : 4096;
+}
+// Polls of a cooperative wait before it gives up ($TRIDIAG_COOP_SPIN).
+static int32_t coop_spin_limit() {
+    static const int32_t v = (int32_t)std::max<int64_t>(0, std::min<int64_t>(
+        env_int("TRIDIAG_COOP_SPIN", (int64_t)1 << 22), 0x7fffffff));
+    return v;
+}
+
+// The entry of the cooperative table for (layout, n): interleaved blocks must
+// divide n; K = blocks per system in [2, kmax].  Contiguous systems take it up to
+// 16 blocks: measured on MI355X at 2^24 unknowns against the chunked solve, FP32
+// 20000 rows 0.090 vs 0.114 ms and 65536 rows 0.094 vs 0.110 ms, FP64 20000 rows
+// 0.184 vs 0.212 ms; with more blocks every group waits on more workgroups and it
+// fell behind (FP64 65536 rows in 32 blocks 0.220 vs 0.208 ms, 2^20 rows 1.6 vs
+// 0.21 ms).
+template <typename T>
+static const CoopEntry<T> *choose_coop(bool il, int64_t n, int &K) {
+    CoopTable<T> tab = coop_table<T>();
+    for (int i = 0; i < tab.count; ++i) {
+        const CoopEntry<T> &e = tab.entries[i];
+        if (e.IL != il) continue;
+        const int64_t k = il ? (n % e.R == 0 ? n / e.R : 0) : (n + e.R - 1) / e.R;
+        if (k < 2 || k > e.kmax) continue;
+        K = (int)k;
+        return &e;
+    }
+    return nullptr;
+}
+
+// How solve_impl solves one batch (decided without launching anything).
+enum PlanKind {
+    kPlanLane = 0,           // one system per lane, in registers (short systems)
+    kPlanThomas = 1,         // the Thomas recurrence (algorithm 1)
+    kPlanTile = 2,           // on chip, one workgroup per system or per S systems
+    kPlanCoop = 3,           // on chip, K workgroups per group of systems
+    kPlanChunked = 4,        // three passes over chunks through global memory
+    kPlanViaContiguous = 5,  // interleaved systems solved through a contiguous copy
+};
+template <typename T> struct SolvePlan {
+    int kind = kPlanTile;
+    const TileEntry<T> *tile = nullptr;
+    const CoopEntry<T> *coop = nullptr;
+    int K = 0;                 // cooperative: row blocks per system
+    int grid = 0;              // cooperative: persistent workgroups
+    size_t forms_bytes = 0;    // cooperative: scratch for the block forms, then counters
+    size_t scratch_bytes = 0;
+};
+
+// Whether the cooperative row-block solve takes (algo, layout, n, batch): the
+// Wang-Mou interface family, an entry whose blocks fit (interleaved: dividing n),
+// 32-bit offsets, and a device that holds at least K of its workgroups at once.
+// fixed_bytes > 0: the scratch must fit a fixed buffer of that size.
+template <typename T>
+static bool coop_plan(int algo, int layout, int64_t n, int64_t batch, size_t fixed_bytes,
+                      SolvePlan<T> &pl) {
+    const int prec = sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE;
+    if (coop_min_rows_il(prec) < 0 || algo_spec(algo).iface != kIfaceWM) return false;
+    const bool il = layout == kInterleaved;
+    if (il && n < coop_min_rows_il(prec)) return false;
+    int K = 0;
+    const CoopEntry<T> *e = choose_coop<T>(il, n, K);
+    if (!e) return false;
+    // interleaved: 32-bit offsets into whole arrays; contiguous: into one system
+    const int64_t elems = il ? n * batch : n;
+    if (elems * (int64_t)sizeof(T) >= (int64_t)0x7fffff00) return false;
+    const int64_t ngroups = (batch + e->S - 1) / e->S;
+    const int64_t ntask = ngroups * K;
+    if (ntask > 0x7fffffff) return false;
+    // every task of a round must be resident: P workgroups, a multiple of K
+    const int cap = e->max_grid();
+    if (cap < K) return false;
+    const int64_t P = std::min<int64_t>(ntask, (cap / K) * (int64_t)K);
+    const size_t forms = (((size_t)ntask * e->S * 6 * sizeof(T)) + 255) & ~(size_t)255;
+    const size_t bytes = forms + (size_t)ngroups * kCoopCtlStride * sizeof(int32_t);
+    if (fixed_bytes && bytes > fixed_bytes) return false;
+    pl.kind = kPlanCoop;
+    pl.coop = e;
+    pl.K = K;
+    pl.grid = (int)P;
+    pl.forms_bytes = forms;
+    pl.scratch_bytes = bytes;
+    return true;
+}
+
+template <typename T>
+static int coop_run(const SolvePlan<T> &pl, int layout, int64_t n, int64_t batch, int64_t sstride,
+                    T *a, T *b, T *c, T *d, hipStream_t st, const FixedScratch *fx) {
+    const bool il = layout == kInterleaved;
+    const CoopEntry<T> &e = *pl.coop;
+    const int64_t ngroups = (batch + e.S - 1) / e.S;
+    void *scratch = nullptr;
+    if (fx) {
+        scratch = fx->ptr;
+    } else {
+        const hipError_t ea = hipMallocAsync(&scratch, pl.scratch_bytes, st);
+        if (ea != hipSuccess) return hip_fail(ea, "cooperative solve scratch");
+    }
+    CoopParams<T> p{a, b, c, d, n, batch, il ? 1 : sstride, il ? batch : 1,
+                    (int32_t)(il ? n * batch * (int64_t)sizeof(T) : 0), (int32_t)pl.K,
+                    (int32_t)ngroups, coop_spin_limit(), reinterpret_cast<T *>(scratch),
+                    reinterpret_cast<int32_t *>(reinterpret_cast<char *>(scratch) +
+                                                pl.forms_bytes)};
+    int rc = TRIDIAG_OK;
+    hipError_t err = launch_coop_reset(p.ctl, (int32_t)(ngroups * kCoopCtlStride), st);
+    if (err == hipSuccess) err = e.launch(p, pl.grid, st);
+    if (err != hipSuccess) rc = hip_fail(err, "cooperative solve launch");
+    if (!fx) {
+        const hipError_t ef = hipFreeAsync(scratch, st);
+        if (rc == TRIDIAG_OK && ef != hipSuccess) rc = hip_fail(ef, "cooperative solve scratch");
+    }
     return rc;
 }
 
@@ -354,7 +511,7 @@ static int convert_impl(int src_layout, int dst_layout, int64_t n, int64_t batch
                         const T *const src[4], T *const dst[4], hipStream_t st);
 template <typename T>
 static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b, T *c, T *d,
-                      hipStream_t st);
+                      hipStream_t st, const FixedScratch *fx = nullptr);
 
 // Interleaved systems longer than the interleaved tiles hold: converted to the
 // contiguous layout in stream-ordered scratch, solved there (the chunked solve for
@@ -394,9 +551,43 @@ template <typename T> static int64_t lane_auto_rows(int layout) {
     return lane_thomas_max_rows(sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE);
 }
 
+// The path solve_impl takes for one batch below 2 GiB per array.  algo may be
+// TRIDIAG_ALGO_AUTO.  vec_ok: rows are 16-byte aligned (contiguous layouts).
+// fixed_bytes > 0: scratch must fit a fixed buffer of that many bytes.
+template <typename T>
+static SolvePlan<T> make_plan(int algo, int layout, int64_t n, int64_t batch, int64_t sstride,
+                              bool vec_ok, size_t fixed_bytes) {
+    SolvePlan<T> pl;
+    const bool il = layout == kInterleaved;
+    const int prec = sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE;
+    const int32_t bb = buffer_extent<T>(n, batch, sstride, layout);
+    // short systems: one system per lane in registers (Thomas), for the Thomas id and
+    // for the automatic choice up to lane_auto_rows()
+    if (bb != 0 && n <= lane_thomas_max_rows(prec) && (il || layout == kContiguous) &&
+        (algo == kThomas || (algo == TRIDIAG_ALGO_AUTO && n <= lane_auto_rows<T>(layout)))) {
+        pl.kind = kPlanLane;
+        return pl;
+    }
+    if (algo == kThomas) {
+        pl.kind = kPlanThomas;
+        return pl;
+    }
+    // rows beyond one workgroup: K workgroups per group of systems, on chip
+    if ((il || n > large_min_rows<T>()) && coop_plan<T>(algo, layout, n, batch, fixed_bytes, pl))
+        return pl;
+    if (!il && n > large_min_rows<T>() && n > large_chunk_rows(prec, 0)) {
+        pl.kind = kPlanChunked;
+        return pl;
+    }
+    pl.tile = choose_tile<T>(algo, layout, n, vec_ok, bb == 0);
+    if (!pl.tile)   // no tile of this algorithm holds n (e.g. CR_reg R, or long interleaved)
+        pl.kind = il ? kPlanViaContiguous : kPlanChunked;
+    return pl;
+}
+
 template <typename T>
 static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b, T *c, T *d,
-                      hipStream_t st) {
+                      hipStream_t st, const FixedScratch *fx) {
     if (n < 1) return fail(TRIDIAG_EINVAL, "n_eqt must be >= 1");
     if (batch < 0) return fail(TRIDIAG_EINVAL, "n_batch must be >= 0");
     if (layout < 0 || layout > 2) return fail(TRIDIAG_EINVAL, "layout must be 0, 1 or 2");
@@ -422,7 +613,7 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
                 int rc = solve_impl<T>(algo, layout, n, cnt, a + o,
                                        layout == kPacked ? nullptr : b + o,
                                        layout == kPacked ? nullptr : c + o,
-                                       layout == kPacked ? nullptr : d + o, st);
+                                       layout == kPacked ? nullptr : d + o, st, fx);
                 if (rc != TRIDIAG_OK) return rc;
             }
             return TRIDIAG_OK;
@@ -431,59 +622,56 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
     if (layout == kPacked) {
         b = a + n; c = a + 2 * n; d = a + 3 * n;
     }
-    // short systems: one system per lane in registers (Thomas), for the Thomas id and
-    // for the automatic choice up to lane_auto_rows()
-    const int32_t lane_bb = buffer_extent<T>(n, batch, sstride, layout);
-    const int prec_id = sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE;
-    if (lane_bb != 0 && n <= lane_thomas_max_rows(prec_id) &&
-        (layout == kInterleaved || layout == kContiguous) &&
-        (algo == kThomas || (algo == TRIDIAG_ALGO_AUTO && n <= lane_auto_rows<T>(layout)))) {
-        hipError_t e = (sizeof(T) == 4)
+    const int V = 16 / (int)sizeof(T);
+    const bool vec_ok = layout != kInterleaved && (sstride % V) == 0 && aligned16<T>(a) &&
+                        aligned16<T>(b) && aligned16<T>(c) && aligned16<T>(d);
+    const SolvePlan<T> pl = make_plan<T>(algo, layout, n, batch, sstride, vec_ok,
+                                         fx ? fx->bytes : 0);
+    const int32_t bb = buffer_extent<T>(n, batch, sstride, layout);
+    const bool il = layout == kInterleaved;
+    switch (pl.kind) {
+    case kPlanLane: {
+        const hipError_t e = (sizeof(T) == 4)
             ? launch_lane_thomas_f32((const float *)a, (const float *)b, (const float *)c, (float *)d,
-                                     n, batch, sstride, rstride, layout == kInterleaved, lane_bb, st)
+                                     n, batch, sstride, rstride, il, bb, st)
             : launch_lane_thomas_f64((const double *)a, (const double *)b, (const double *)c,
-                                     (double *)d, n, batch, sstride, rstride, layout == kInterleaved,
-                                     lane_bb, st);
+                                     (double *)d, n, batch, sstride, rstride, il, bb, st);
         return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "lane Thomas launch");
     }
-    if (algo == kThomas) {
-        if (layout != kInterleaved && lane_bb != 0) {
+    case kPlanThomas: {
+        if (!il && bb != 0) {
             // lane groups per system, rows as 16-byte vectors (aligned rows below 2 GiB)
-            hipError_t e = (sizeof(T) == 4)
+            const hipError_t e = (sizeof(T) == 4)
                 ? launch_thomas_rows_f32((const float *)a, (const float *)b, (float *)c, (float *)d,
-                                         n, batch, sstride, lane_bb, st)
+                                         n, batch, sstride, bb, st)
                 : launch_thomas_rows_f64((const double *)a, (const double *)b, (double *)c,
-                                         (double *)d, n, batch, sstride, lane_bb, st);
+                                         (double *)d, n, batch, sstride, bb, st);
             if (e != hipErrorInvalidValue)
                 return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "Thomas launch");
         }
-        hipError_t e = (sizeof(T) == 4)
+        const hipError_t e = (sizeof(T) == 4)
             ? launch_thomas_f32((const float *)a, (const float *)b, (float *)c, (float *)d, n,
                                 batch, sstride, rstride, st)
             : launch_thomas_f64((const double *)a, (const double *)b, (double *)c, (double *)d, n,
                                 batch, sstride, rstride, st);
         return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "Thomas launch");
     }
-    const int V = 16 / (int)sizeof(T);
-    const bool vec_ok = layout != kInterleaved && (sstride % V) == 0 && aligned16<T>(a) &&
-                        aligned16<T>(b) && aligned16<T>(c) && aligned16<T>(d);
-    const int32_t bb = buffer_extent<T>(n, batch, sstride, layout);
-    if (layout != kInterleaved && n > large_min_rows<T>() &&
-        n > large_chunk_rows(sizeof(T) == 4 ? TRIDIAG_SINGLE : TRIDIAG_DOUBLE, 0))
-        return large_impl<T>(algo, n, batch, sstride, a, b, c, d, st);
-    const TileEntry<T> *e = choose_tile<T>(algo, layout, n, vec_ok, bb == 0);
-    if (!e && layout != kInterleaved)   // no tile of this algorithm holds n (e.g. CR_reg R)
-        return large_impl<T>(algo, n, batch, sstride, a, b, c, d, st);
-    if (!e && layout == kInterleaved)      // no interleaved tile takes it
+    case kPlanCoop:
+        return coop_run<T>(pl, layout, n, batch, sstride, a, b, c, d, st, fx);
+    case kPlanChunked:
+        return large_impl<T>(algo, n, batch, sstride, a, b, c, d, st, fx);
+    case kPlanViaContiguous:
+        if (fx)
+            return fail(TRIDIAG_EUNSUPPORTED,
+                        "interleaved n_eqt=" + std::to_string(n) + " is solved through a "
+                        "contiguous copy of all four arrays, which needs an allocation");
         return interleaved_via_contiguous<T>(algo, n, batch, a, b, c, d, st);
-    if (!e) {
-        return fail(TRIDIAG_EUNSUPPORTED,
-                    "n_eqt=" + std::to_string(n) + " exceeds the on-chip capacity of algorithm " +
-                        std::to_string(algo) + " in layout " + std::to_string(layout));
+    default:
+        break;
     }
     const AlgoSpec sp = algo_spec(algo);
     TileParams<T> p{a, b, c, d, n, batch, sstride, rstride, bb, sp.iface, sp.xchg_lds};
-    hipError_t err = e->launch(p, st);
+    const hipError_t err = pl.tile->launch(p, st);
     if (err == hipErrorInvalidConfiguration)
         return fail(TRIDIAG_EUNSUPPORTED, "interface solver of algorithm " + std::to_string(algo) +
                                               " needs more LDS than the kernel for n_eqt=" +
@@ -948,8 +1136,7 @@ int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_
     g_last_error.clear();
     if (precision != TRIDIAG_SINGLE && precision != TRIDIAG_DOUBLE)
         return fail(TRIDIAG_EINVAL, "precision must be 0 (single) or 1 (double)");
-    if (algo == TRIDIAG_ALGO_AUTO) algo = kWM5Reg;
-    if (algo < 0 || algo >= algo_count(precision))
+    if (algo != TRIDIAG_ALGO_AUTO && (algo < 0 || algo >= algo_count(precision)))
         return fail(TRIDIAG_EINVAL, "unknown algorithm id " + std::to_string(algo));
     if (algo == kVendor) {
         if (layout != kContiguous) return fail(TRIDIAG_EUNSUPPORTED, "gtsv needs layout 0");
@@ -978,29 +1165,67 @@ int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_
                               (double *)c, (double *)d, st);
 }
 
+}  // extern "C"
+
+namespace tri {
+// Plan of tridiag_solve for arrays with 16-byte aligned rows (no launch).
+template <typename T>
+static int plan_impl(int algo, int layout, int64_t n, int64_t batch, SolvePlan<T> &pl) {
+    if (algo != TRIDIAG_ALGO_AUTO && (algo < 0 || algo >= algo_count(sizeof(T) == 4 ? 0 : 1)))
+        return fail(TRIDIAG_EINVAL, "unknown algorithm id " + std::to_string(algo));
+    if (n < 1 || batch < 1 || layout < 0 || layout > 2) return fail(TRIDIAG_EINVAL, "bad shape");
+    if (algo == kVendor) return fail(TRIDIAG_EUNSUPPORTED, "the vendor comparator has no plan");
+    const int64_t sstride = layout == kPacked ? 4 * n : (layout == kInterleaved ? 1 : n);
+    int64_t b = batch;
+    if (layout != kInterleaved && algo != kThomas && buffer_extent<T>(n, b, sstride, layout) == 0)
+        b = std::max<int64_t>(1, ((int64_t)0x7fffff00 / (int64_t)sizeof(T)) / sstride);
+    pl = make_plan<T>(algo, layout, n, b, sstride, layout != kInterleaved, 0);
+    return TRIDIAG_OK;
+}
+}  // namespace tri
+
+extern "C" {
+
 int tridiag_plan(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch, int *M,
                  int *NW, int *S) {
-    (void)n_batch;
-    if (algo == TRIDIAG_ALGO_AUTO) algo = kWM5Reg;
-    if (algo <= kThomas) return fail(TRIDIAG_EUNSUPPORTED, "no tile plan for this algorithm");
-    const bool chunked = layout != 2 && n_eqt > (precision == TRIDIAG_SINGLE
-                                                     ? large_min_rows<float>()
-                                                     : large_min_rows<double>());
-    if (chunked) return fail(TRIDIAG_EUNSUPPORTED, "n_eqt takes the chunked solve, not a tile");
-    int m = 0, nw = 0, s = 0;
-    bool found = false;
+    g_last_error.clear();
+    int m = 0, nw = 0, s = 0, kind = -1;
     if (precision == TRIDIAG_SINGLE) {
-        const TileEntry<float> *e = choose_tile<float>(algo, layout, n_eqt, layout != 2);
-        if (e) { m = e->M; nw = e->NW; s = e->S; found = true; }
+        SolvePlan<float> pl;
+        if (const int rc = plan_impl<float>(algo, layout, n_eqt, n_batch, pl)) return rc;
+        kind = pl.kind;
+        if (pl.tile) { m = pl.tile->M; nw = pl.tile->NW; s = pl.tile->S; }
+        if (pl.coop) { m = pl.coop->M; nw = pl.coop->NW; s = pl.coop->S; }
+    } else if (precision == TRIDIAG_DOUBLE) {
+        SolvePlan<double> pl;
+        if (const int rc = plan_impl<double>(algo, layout, n_eqt, n_batch, pl)) return rc;
+        kind = pl.kind;
+        if (pl.tile) { m = pl.tile->M; nw = pl.tile->NW; s = pl.tile->S; }
+        if (pl.coop) { m = pl.coop->M; nw = pl.coop->NW; s = pl.coop->S; }
     } else {
-        const TileEntry<double> *e = choose_tile<double>(algo, layout, n_eqt, layout != 2);
-        if (e) { m = e->M; nw = e->NW; s = e->S; found = true; }
+        return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
     }
-    if (!found) return fail(TRIDIAG_EUNSUPPORTED, "n_eqt exceeds on-chip capacity");
+    if (kind != kPlanTile && kind != kPlanCoop)
+        return fail(TRIDIAG_EUNSUPPORTED, "this shape is not solved by an on-chip workgroup plan "
+                                          "(see tridiag_plan_kind)");
     if (M) *M = m;
     if (NW) *NW = nw;
     if (S) *S = s;
     return TRIDIAG_OK;
+}
+
+int tridiag_plan_kind(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch) {
+    g_last_error.clear();
+    if (precision == TRIDIAG_SINGLE) {
+        SolvePlan<float> pl;
+        return plan_impl<float>(algo, layout, n_eqt, n_batch, pl) ? -1 : pl.kind;
+    }
+    if (precision == TRIDIAG_DOUBLE) {
+        SolvePlan<double> pl;
+        return plan_impl<double>(algo, layout, n_eqt, n_batch, pl) ? -1 : pl.kind;
+    }
+    fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
+    return -1;
 }
 
 int tridiag_tile_count(int precision) {

@@ -309,6 +309,89 @@ template <typename T, int CH, int V> struct SafeLoader {
     }
 };
 
+// Interleaved whole-piece loads ("XP-IL"): rows [wrow, wrow + SW*M) of the S
+// consecutive systems starting at col0, where every row piece is S*sizeof(T)
+// contiguous bytes (CPR 16-byte chunks).  Consecutive lanes read consecutive
+// chunks (il_block_issue, NQ = M*sizeof(T)/16 chunks per lane and array); lane
+// (system sl, segment kw) then takes its M rows of system sl from the wave's LDS
+// block xb (il_block_transpose).  Offsets are 32-bit: the arrays are below 2 GiB.
+template <typename T, int M, int SW, int S, bool NT>
+__device__ __forceinline__ void il_block_issue(const __amdgpu_buffer_rsrc_t (&rr)[4], int64_t wrow,
+                                               int64_t rs, int64_t col0, int lane,
+                                               u32x4 (&g)[4][M * (int)sizeof(T) / 16]) {
+    constexpr int CPR = S * (int)sizeof(T) / 16;
+    constexpr int RB = SW * M;
+    constexpr int NQ = RB * CPR / 64;
+    static_assert((RB * CPR) % 64 == 0 && NQ == M * (int)sizeof(T) / 16,
+                  "wave block must be whole 16-byte loads");
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int cidx = q * 64 + lane, r = cidx / CPR, h = cidx % CPR;
+        const int off = (int)(((wrow + r) * rs + col0) * (int64_t)sizeof(T)) + h * 16;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            g[k][q] = __builtin_amdgcn_raw_buffer_load_b128(rr[k], off, 0, cpol<NT>());
+    }
+}
+
+template <typename T, int M, int S>
+__device__ __forceinline__ void il_block_transpose(const u32x4 (&g)[4][M * (int)sizeof(T) / 16],
+                                                   int lane, int sl, int kw, u32x4 *xb,
+                                                   T (&rows)[4][M]) {
+    constexpr int NQ = M * (int)sizeof(T) / 16;
+    const T *tb = reinterpret_cast<const T *>(xb);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k > 0) wave_lds_sync();
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) xb[q * 64 + lane] = g[k][q];
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < M; ++j) rows[k][j] = tb[(kw * M + j) * S + sl];
+    }
+}
+
+template <typename T, int M, int SW, int S, bool NT>
+__device__ __forceinline__ void il_block_load(const __amdgpu_buffer_rsrc_t (&rr)[4], int64_t wrow,
+                                              int64_t rs, int64_t col0, int lane, int sl, int kw,
+                                              u32x4 *xb, T (&rows)[4][M]) {
+    u32x4 g[4][M * (int)sizeof(T) / 16];
+    il_block_issue<T, M, SW, S, NT>(rr, wrow, rs, col0, lane, g);
+    il_block_transpose<T, M, S>(g, lane, sl, kw, xb, rows);
+}
+
+// Stores x of the same block (xs[j] = x of the lane's row j) the same way; chunks
+// of systems at or past `batch` are not written.  Every lane of the wave takes part.
+template <typename T, int M, int SW, int S, bool NT>
+__device__ __forceinline__ void il_block_store(__amdgpu_buffer_rsrc_t rd, int64_t wrow, int64_t rs,
+                                               int64_t col0, int64_t batch, int lane, int sl,
+                                               int kw, u32x4 *xb, const T (&xs)[M]) {
+    constexpr int VV = 16 / (int)sizeof(T);
+    constexpr int CPR = S * (int)sizeof(T) / 16;
+    constexpr int RB = SW * M;
+    constexpr int NQ = RB * CPR / 64;
+    T *tb = reinterpret_cast<T *>(xb);
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < M; ++j) tb[(kw * M + j) * S + sl] = xs[j];
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int cidx = q * 64 + lane, r = cidx / CPR, h = cidx % CPR;
+        const int off = (int)(((wrow + r) * rs + col0) * (int64_t)sizeof(T)) + h * 16;
+        const int64_t s0 = col0 + h * VV;                  // first system of the chunk
+        const u32x4 t = xb[cidx];
+        if (s0 + VV <= batch) {
+            __builtin_amdgcn_raw_buffer_store_b128(t, rd, off, 0, cpol<NT>());
+        } else {
+#pragma unroll
+            for (int v = 0; v < VV; ++v)
+                if (s0 + v < batch)
+                    bstore<T, NT>(reinterpret_cast<const T *>(&t)[v], rd, off + v * (int)sizeof(T), 0);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- lane sweep
 // Forward modified-Thomas over the lane's interior rows j = 1..M-2 with x_L kept
 // symbolic.  Leaves x_j = de_j - al_j x_L - ga_j x_{j+1} for every interior row
@@ -578,38 +661,12 @@ __global__ void __launch_bounds__(64 * NW)
         __amdgpu_buffer_rsrc_t ra = make_rsrc(prm.a, nb), rb = make_rsrc(prm.b, nb),
                                rc = make_rsrc(prm.c, nb);
         if constexpr (XP && IL) {
-            // wave block: rows [ws*SW*M, (ws+1)*SW*M) of the tile's S systems; each row
-            // piece is S*sizeof(T) contiguous bytes = CPR 16-byte chunks, read by
-            // consecutive lanes; lane (sl, kw) then takes column sl of its M rows
-            constexpr int VV = 16 / (int)sizeof(T);
-            constexpr int CPR = S * (int)sizeof(T) / 16;
-            constexpr int RB = SW * M;
-            constexpr int NQ = RB * CPR / 64;
-            static_assert((RB * CPR) % 64 == 0, "wave block must be whole 16-byte loads");
-            const int64_t wrow = (int64_t)ws * RB;
-            const int64_t col0 = (int64_t)tile * S;
-            u32x4 g[4][NQ];
+            // wave block: rows [ws*SW*M, (ws+1)*SW*M) of the tile's S systems, read as
+            // whole row pieces and handed to the lanes through LDS
             const __amdgpu_buffer_rsrc_t rr[4] = {ra, rb, rc, rd_rsrc};
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const int cidx = q * 64 + lane, r = cidx / CPR, h = cidx % CPR;
-                const int off = (int)(((wrow + r) * rs + col0) * (int64_t)sizeof(T)) + h * 16;
-#pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    g[k][q] = __builtin_amdgcn_raw_buffer_load_b128(rr[k], off, 0, cpol<NT>());
-            }
-            (void)VV;
             T rows[4][M];
-            const T *tb = reinterpret_cast<const T *>(xbuf[wave]);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (k > 0) wave_lds_sync();
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) xbuf[wave][q * 64 + lane] = g[k][q];
-                wave_lds_sync();
-#pragma unroll
-                for (int j = 0; j < M; ++j) rows[k][j] = tb[(kw * M + j) * S + sl];
-            }
+            il_block_load<T, M, SW, S, NT>(rr, (int64_t)ws * SW * M, rs, (int64_t)tile * S, lane,
+                                           sl, kw, xbuf[wave], rows);
             RegLoader<T, CH, M> ld{rows};
             lane_sweep<T, Sh>(o, ld, row0, n, al_store);
         } else if constexpr (XP) {
@@ -697,35 +754,12 @@ __global__ void __launch_bounds__(64 * NW)
     }
     if constexpr (XP && IL) {
         // every lane takes part: chunk stores are assigned by position, not by system
-        constexpr int VV = 16 / (int)sizeof(T);
-        constexpr int CPR = S * (int)sizeof(T) / 16;
-        constexpr int RB = SW * M;
-        constexpr int NQ = RB * CPR / 64;
-        const int64_t wrow = (int64_t)ws * RB;
-        const int64_t col0 = (int64_t)tile * S;
-        T *tb = reinterpret_cast<T *>(xbuf[wave]);
-        wave_lds_sync();
+        T xs[M];
 #pragma unroll
         for (int j = 0; j < M; ++j)
-            tb[(kw * M + j) * S + sl] =
-                (j == 0) ? xL : ((j == M - 1) ? xR : de[(j >= 1 && j <= M - 2) ? j - 1 : 0]);
-        wave_lds_sync();
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int cidx = q * 64 + lane, r = cidx / CPR, h = cidx % CPR;
-            const int off = (int)(((wrow + r) * rs + col0) * (int64_t)sizeof(T)) + h * 16;
-            const int64_t s0 = col0 + h * VV;                  // first system of the chunk
-            const u32x4 t = xbuf[wave][cidx];
-            if (s0 + VV <= prm.batch) {
-                __builtin_amdgcn_raw_buffer_store_b128(t, rd_rsrc, off, 0, cpol<NT>());
-            } else {
-#pragma unroll
-                for (int v = 0; v < VV; ++v)
-                    if (s0 + v < prm.batch)
-                        bstore<T, NT>(reinterpret_cast<const T *>(&t)[v], rd_rsrc,
-                                      off + v * (int)sizeof(T), 0);
-            }
-        }
+            xs[j] = (j == 0) ? xL : ((j == M - 1) ? xR : de[(j >= 1 && j <= M - 2) ? j - 1 : 0]);
+        il_block_store<T, M, SW, S, NT>(rd_rsrc, (int64_t)ws * SW * M, rs, (int64_t)tile * S,
+                                        prm.batch, lane, sl, kw, xbuf[wave], xs);
         return;
     }
     if (!valid) return;
