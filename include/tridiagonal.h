@@ -134,6 +134,28 @@ int tridiag_algo_layout(int precision, int algo);
 int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch,
                   void *a, void *b, void *c, void *d, void *stream);
 
+/* tridiag_solve without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and error behaviour as tridiag_solve, except:
+ *   - the paths of tridiag_solve whose kernels use scratch memory (the
+ *     cooperative row-block solve and the chunked solve of long systems) keep it
+ *     in a fixed device array of TRIDIAG_NOALLOC_SCRATCH_BYTES per GPU, owned by
+ *     the library and held until the process ends, instead of stream-ordered
+ *     allocations; the chunked solve then runs in as many launch groups as the
+ *     array needs, and a cooperative solve whose scratch exceeds it takes the
+ *     path tridiag_solve would take without it (its results then agree with
+ *     tridiag_solve's to the solver's tolerance rather than bit for bit);
+ *   - algorithm 0 (the vendor comparator, which allocates its own buffer) and
+ *     interleaved systems longer than the interleaved tiles hold (solved through
+ *     a contiguous copy of all four arrays) return TRIDIAG_EUNSUPPORTED.
+ * Calls on one GPU share its array: calls that overlap on one GPU (another
+ * stream, another thread) race.  The first call on a GPU looks up the array's
+ * device address (hipGetSymbolAddress). */
+#define TRIDIAG_NOALLOC_SCRATCH_BYTES ((size_t)4 << 20)
+int tridiag_solve_noalloc(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch,
+                          void *a, void *b, void *c, void *d, void *stream);
+
 /* Ragged batch: systems of uneven size, contiguous per system and concatenated.
  * System s holds rows [offsets[s], offsets[s+1]) of a, b, c, d (a_0 and c_{n-1}
  * of each system are ignored); x replaces d.  Sizes 0..8192 rows.  The systems

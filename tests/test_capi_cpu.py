@@ -73,6 +73,10 @@ def test_argument_validation_without_gpu(capi):
     assert L.tridiag_solve(0, 0, 2, 16, 4, nul, nul, nul, nul, nul) == capi.EUNSUPPORTED
     assert L.tridiag_solve(1, 0, 0, 2, 4, nul, nul, nul, nul, nul) == capi.EUNSUPPORTED
     assert b">= 3" in L.tridiag_last_error()
+    # the no-allocation solve refuses the vendor comparator before touching a GPU
+    assert L.tridiag_solve_noalloc(0, 0, 0, 16, 4, nul, nul, nul, nul, nul) == capi.EUNSUPPORTED
+    assert L.tridiag_solve_noalloc(7, 9, 0, 16, 4, nul, nul, nul, nul, nul) == capi.EINVAL
+    assert L.tridiag_solve_noalloc(0, 99, 0, 16, 4, nul, nul, nul, nul, nul) == capi.EINVAL
     # a ragged solve needs a work buffer planned in this process
     fake = ctypes.c_void_p(0x1000)
     assert L.tridiag_ragged_solve(0, 4, fake, fake, fake, fake, fake, nul) == capi.EINVAL

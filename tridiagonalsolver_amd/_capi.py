@@ -63,6 +63,8 @@ def lib():
     L.tridiag_algo_layout.restype = ci
     L.tridiag_solve.argtypes = [ci, ci, ci, i64, i64, vp, vp, vp, vp, vp]
     L.tridiag_solve.restype = ci
+    L.tridiag_solve_noalloc.argtypes = [ci, ci, ci, i64, i64, vp, vp, vp, vp, vp]
+    L.tridiag_solve_noalloc.restype = ci
     L.tridiag_ragged_work_size.argtypes = [i64]
     L.tridiag_ragged_work_size.restype = i64
     L.tridiag_solve_ragged.argtypes = [ci, i64, vp, vp, vp, vp, vp, vp, vp]

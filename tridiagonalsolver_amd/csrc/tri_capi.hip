@@ -679,6 +679,44 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
     return err == hipSuccess ? TRIDIAG_OK : hip_fail(err, "tile launch");
 }
 
+// ---------------------------------------------------------------- no-allocation solve
+// Scratch of tridiag_solve_noalloc: the chunked solve's per-chunk outer forms and
+// boundary values (8 words per chunk of every system of one launch group), or the
+// cooperative solve's block forms and group counters.  One copy per GPU, at a
+// device address of its own, held until the process ends.  Every
+// tridiag_solve_noalloc call on a GPU shares that GPU's copy; calls that overlap
+// on one GPU race.  The chunked solve writes every word it reads; the cooperative
+// solve clears its counters with a kernel before each use, so the contents left
+// by an earlier call need no clearing.
+constexpr size_t kNoAllocScratchBytes = TRIDIAG_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[kMaxDevices];
+
+// Device address of the current GPU's `scratch`, looked up on the first call on
+// that GPU.  (The array's name is a host-side stand-in; only this address may be
+// handed to kernels.)
+static int noalloc_scratch(void **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(TRIDIAG_EUNSUPPORTED, "device index " + std::to_string(dev) + " beyond " +
+                                              std::to_string(kMaxDevices));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail(TRIDIAG_EHIP, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_dev[dev] = ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return TRIDIAG_OK;
+}
+
 // ---------------------------------------------------------------- vendor comparator
 static hipsparseHandle_t g_sparse = nullptr;
 static void *g_sparse_buf = nullptr;
@@ -1163,6 +1201,27 @@ int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_
                                  (float *)c, (float *)d, st);
     return solve_impl<double>(algo, layout, n_eqt, n_batch, (double *)a, (double *)b,
                               (double *)c, (double *)d, st);
+}
+
+int tridiag_solve_noalloc(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch,
+                          void *a, void *b, void *c, void *d, void *stream) {
+    g_last_error.clear();
+    if (precision != TRIDIAG_SINGLE && precision != TRIDIAG_DOUBLE)
+        return fail(TRIDIAG_EINVAL, "precision must be 0 (single) or 1 (double)");
+    if (algo != TRIDIAG_ALGO_AUTO && (algo < 0 || algo >= algo_count(precision)))
+        return fail(TRIDIAG_EINVAL, "unknown algorithm id " + std::to_string(algo));
+    if (algo == kVendor)
+        return fail(TRIDIAG_EUNSUPPORTED, "the vendor gtsv comparator (algorithm 0) allocates its "
+                                          "own buffer; tridiag_solve_noalloc does not run it");
+    void *ptr = nullptr;
+    if (const int rc = noalloc_scratch(&ptr)) return rc;
+    const FixedScratch fx{ptr, kNoAllocScratchBytes};
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == TRIDIAG_SINGLE)
+        return solve_impl<float>(algo, layout, n_eqt, n_batch, (float *)a, (float *)b,
+                                 (float *)c, (float *)d, st, &fx);
+    return solve_impl<double>(algo, layout, n_eqt, n_batch, (double *)a, (double *)b,
+                              (double *)c, (double *)d, st, &fx);
 }
 
 }  // extern "C"
