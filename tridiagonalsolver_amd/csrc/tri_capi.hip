@@ -831,19 +831,6 @@ static int solve_tile_impl(int index, int iface, int layout, int64_t n, int64_t 
 
 
 // ---------------------------------------------------------------- ragged batches
-// Default shape of a system of n rows: the smallest capacity that holds it; among
-// equal capacities the one with the fewest waves per system (no cross-wave chain).
-static int ragged_default_shape(int64_t n) {
-    int best = -1;
-    for (int k = 0; k < kRaggedShapes; ++k) {
-        if (ragged_cap(k) < n) continue;
-        if (best < 0 || ragged_cap(k) < ragged_cap(best) ||
-            (ragged_cap(k) == ragged_cap(best) && kRaggedShapeTab[k].NWS < kRaggedShapeTab[best].NWS))
-            best = k;
-    }
-    return best;
-}
-
 // Bucket choice for a system of n rows: the autotuned row of its size bucket,
 // else the default shape with the Wang-Mou interface solver.
 static void ragged_choice(int prec, int64_t n, int &shape, int &iface) {
@@ -898,95 +885,20 @@ static int ragged_plan_check(const void *work, int64_t n_sys, int prec) {
     return fail(TRIDIAG_EINVAL, "the work buffer holds no plan (call tridiag_ragged_plan first)");
 }
 
-static int64_t ragged_work_bytes(int64_t n_sys) {
-    const int64_t raw = kRaggedHeaderBytes + (int64_t)kRaggedRecInts * 4 * n_sys;
-    return (raw + 255) & ~(int64_t)255;
-}
-
-// Host planner: validates the offsets, buckets the systems, writes one record per
-// work item (largest bucket first) and copies the schedule into `work`.
+// Host planner (ragged_schedule: validates the offsets, buckets the systems, one
+// record per work item, largest bucket first) and the upload into `work`.
 static int ragged_plan_impl(int prec, int64_t n_sys, const int64_t *off, int force_shape,
                             int force_iface, void *work, hipStream_t st) {
-    if (prec != TRIDIAG_SINGLE && prec != TRIDIAG_DOUBLE)
-        return fail(TRIDIAG_EINVAL, "precision must be 0 or 1");
-    if (n_sys < 0 || n_sys > 0x7fffffff) return fail(TRIDIAG_EINVAL, "n_sys out of range");
     if (!work) return fail(TRIDIAG_EINVAL, "null work buffer");
-    if (n_sys > 0 && !off) return fail(TRIDIAG_EINVAL, "null offsets");
-    if (force_shape >= kRaggedShapes || force_shape < -1)
-        return fail(TRIDIAG_EINVAL, "shape index out of range");
-    if (force_iface > kIfaceCR || force_iface < -1)
-        return fail(TRIDIAG_EINVAL, "interface solver id out of range");
-    if (n_sys > 0 && off[0] < 0) return fail(TRIDIAG_EINVAL, "offsets[0] is negative");
-    // (shape, iface) bucket of every system
-    constexpr int NF = kIfaceCR + 1;
-    std::vector<int32_t> key((size_t)n_sys, -1);
-    std::vector<int64_t> cnt((size_t)kRaggedShapes * NF, 0);
-    for (int64_t s = 0; s < n_sys; ++s) {
-        const int64_t n = off[s + 1] - off[s];
-        if (n < 0) return fail(TRIDIAG_EINVAL, "offsets must be non-decreasing");
-        if (n == 0) continue;
-        if (n > kRaggedMaxN)
-            return fail(TRIDIAG_EUNSUPPORTED, "system " + std::to_string(s) + " has " +
-                                                  std::to_string(n) + " rows; the ragged solve "
-                                                  "holds up to " + std::to_string(kRaggedMaxN));
-        int shape, iface;
-        ragged_choice(prec, n, shape, iface);
-        if (force_shape >= 0) {
-            if (ragged_cap(force_shape) < n)
-                return fail(TRIDIAG_EINVAL, "system " + std::to_string(s) +
-                                                " exceeds the capacity of the forced shape");
-            shape = force_shape;
-        }
-        if (force_iface >= 0) iface = force_iface;
-        key[(size_t)s] = shape * NF + iface;
-        ++cnt[(size_t)key[(size_t)s]];
-    }
-    // buckets in order of decreasing capacity (then decreasing systems per item)
-    std::vector<int> order(cnt.size());
-    for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
-    std::stable_sort(order.begin(), order.end(), [](int x, int y) {
-        const int sx = x / NF, sy = y / NF;
-        if (ragged_cap(sx) != ragged_cap(sy)) return ragged_cap(sx) > ragged_cap(sy);
-        return kRaggedShapeTab[sx].S < kRaggedShapeTab[sy].S;
-    });
-    // systems of each bucket, in input order
-    std::vector<int64_t> first(cnt.size(), 0);
-    int64_t pos = 0;
-    for (int k : order) {
-        first[(size_t)k] = pos;
-        pos += cnt[(size_t)k];
-    }
-    std::vector<int32_t> perm((size_t)pos);
-    {
-        std::vector<int64_t> fill(first);
-        for (int64_t s = 0; s < n_sys; ++s)
-            if (key[(size_t)s] >= 0) perm[(size_t)fill[(size_t)key[(size_t)s]]++] = (int32_t)s;
-    }
-    const int64_t bytes = ragged_work_bytes(n_sys);
-    std::vector<char> host((size_t)bytes, 0);
-    int32_t *recs = reinterpret_cast<int32_t *>(host.data() + kRaggedHeaderBytes);
-    int64_t nitems = 0;
-    for (int k : order) {
-        const int S = kRaggedShapeTab[k / NF].S;
-        for (int64_t i = 0; i < cnt[(size_t)k]; i += S) {
-            int32_t *r = recs + (size_t)kRaggedRecInts * nitems++;
-            const int count = (int)std::min<int64_t>(S, cnt[(size_t)k] - i);
-            r[0] = k / NF;
-            r[1] = k % NF;
-            r[2] = count;
-            for (int q = 0; q < count; ++q) {
-                const int64_t s = perm[(size_t)(first[(size_t)k] + i + q)];
-                const uint64_t o = (uint64_t)off[s];
-                r[4 + 3 * q] = (int32_t)(uint32_t)(o & 0xffffffffu);
-                r[5 + 3 * q] = (int32_t)(uint32_t)(o >> 32);
-                r[6 + 3 * q] = (int32_t)(off[s + 1] - off[s]);
-            }
-        }
-    }
-    RaggedHeader h{kRaggedMagic, n_sys, nitems, 0, prec, 0, n_sys > 0 ? off[n_sys] : 0};
-    memcpy(host.data(), &h, sizeof h);
-    const size_t used = (size_t)(kRaggedHeaderBytes + (int64_t)kRaggedRecInts * 4 * nitems);
-    hipError_t e = hipMemcpyAsync(work, host.data(), used, hipMemcpyHostToDevice, st);
+    std::vector<char> host;
+    std::string err;
+    const int rc = ragged_schedule(prec, n_sys, off, force_shape, force_iface,
+                                   [prec](int64_t n, int &shape, int &iface) {
+                                       ragged_choice(prec, n, shape, iface);
+                                   },
+                                   host, err);
+    if (rc) return fail(rc == 2 ? TRIDIAG_EUNSUPPORTED : TRIDIAG_EINVAL, err);
+    hipError_t e = hipMemcpyAsync(work, host.data(), host.size(), hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "ragged plan upload");
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "ragged plan upload");
@@ -1129,22 +1041,9 @@ static int convert_impl(int src_layout, int dst_layout, int64_t n, int64_t batch
     }
     if (count == 0) return fail(TRIDIAG_EINVAL, "no coefficient array to convert");
     if (batch == 0) return TRIDIAG_OK;
-    const bool src_sys = src_layout != kInterleaved, dst_sys = dst_layout != kInterleaved;
-    const int64_t ps = src_layout == kPacked ? 4 * n : n;
-    const int64_t pd = dst_layout == kPacked ? 4 * n : n;
-    bool transpose = false;
-    if (src_sys && dst_sys) {              // system-major to system-major: pitched copy
-        p.rows = batch; p.cols = n; p.ld_src = ps; p.ld_dst = pd;
-        if (ps == n && pd == n) { p.rows = 1; p.cols = elems; p.ld_src = p.ld_dst = elems; }
-    } else if (src_sys) {                  // to interleaved: transpose of batch x n
-        p.rows = batch; p.cols = n; p.ld_src = ps; p.ld_dst = batch;
-        transpose = true;
-    } else if (dst_sys) {                  // from interleaved: transpose of n x batch
-        p.rows = n; p.cols = batch; p.ld_src = batch; p.ld_dst = pd;
-        transpose = true;
-    } else {                               // interleaved to interleaved: flat copy
-        p.rows = 1; p.cols = elems; p.ld_src = p.ld_dst = elems;
-    }
+    const LayoutGeometry geo = layout_geometry(src_layout, dst_layout, n, batch);
+    p.rows = geo.rows; p.cols = geo.cols; p.ld_src = geo.ld_src; p.ld_dst = geo.ld_dst;
+    const bool transpose = geo.transpose;
     hipError_t e;
     if constexpr (sizeof(T) == 4) e = launch_layout_move_f32(p, count, transpose, st);
     else e = launch_layout_move_f64(p, count, transpose, st);

@@ -14,46 +14,9 @@
 // workgroup leaves the loop once it draws an index past the end of the list.
 #pragma once
 #include "tri_host.hpp"
+#include "tri_ragged_plan.hpp"
 
 namespace tri {
-
-// Shapes of the ragged kernel (512 lanes): {M, NWS, S}; capacity 64*NWS*M rows.
-struct RaggedShape {
-    int M, NWS, S;
-};
-constexpr int kRaggedShapes = 16;
-constexpr RaggedShape kRaggedShapeTab[kRaggedShapes] = {
-    {2, 1, 8},  {4, 1, 8},  {8, 1, 8},  {8, 2, 4},  {16, 1, 8}, {8, 4, 2},
-    {16, 2, 4}, {8, 8, 1},  {16, 4, 2}, {16, 8, 1}, {4, 2, 4},  {4, 4, 2},
-    // 12 rows per lane: capacities 1.5x a power of two, so that systems between
-    // two powers of two leave fewer padding rows
-    {12, 1, 8}, {12, 2, 4}, {12, 4, 2}, {12, 8, 1}};
-constexpr int ragged_cap(int k) { return 64 * kRaggedShapeTab[k].NWS * kRaggedShapeTab[k].M; }
-constexpr int kRaggedLanes = 512;
-constexpr int kRaggedWaves = kRaggedLanes / 64;
-constexpr int kRaggedMaxN = 64 * 8 * 16;
-constexpr int kRaggedMaxM = 16;
-
-// Work buffer: a 64-byte header, then one 128-byte record per item:
-//   r[0] shape, r[1] interface solver, r[2] number of systems, r[3] 0,
-//   r[4 + 3k], r[5 + 3k]  first row of system slot k (int64, low word first),
-//   r[6 + 3k]             rows of system slot k.
-// Written by the host planner once (with both counters zero).  A solve leaves the
-// header as it found it: the last workgroup to finish zeroes both counters.
-struct RaggedHeader {
-    int64_t magic;
-    int64_t n_sys;
-    int64_t nitems;
-    int32_t counter;     // work-queue head
-    int32_t precision;
-    int32_t done;        // workgroups of the running solve that have left the queue
-    int64_t rows;        // offsets[n_sys]: rows of the arrays that the systems cover
-};
-constexpr int64_t kRaggedMagic = 0x52414747454432LL;
-constexpr int64_t kRaggedHeaderBytes = 64;
-static_assert(sizeof(RaggedHeader) <= kRaggedHeaderBytes, "header exceeds its 64 bytes");
-constexpr int kRaggedRecInts = 32;
-constexpr int kRaggedMaxSlots = 8;
 
 template <typename T> struct RaggedParams {
     const T *a, *b, *c;
