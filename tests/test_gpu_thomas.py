@@ -46,16 +46,18 @@ def test_thomas_low_margin_long(prec):
     assert st["nonfinite"] == 0 and st["relerr"] <= TOL[DT[prec]], st
 
 
-# rows per step: 16 lanes x one 16-byte vector; prefetch rings of 4 steps (forward)
-# and 8 steps (backward)
-_STEP = {0: 64, 1: 32}
+# rows per step: 64 lanes x one 16-byte vector (the scan kernel) and 16 lanes x one
+# vector (the team kernel); prefetch rings of 4 steps (forward) and 8 steps (backward)
+_STEP = {0: (256, 64), 1: (128, 32)}
 
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_thomas_step_boundaries(prec):
-    sr = _STEP[prec]
-    sizes = sorted({17, 33, sr - 1, sr, sr + 1, 2 * sr + 3, 4 * sr - 1, 4 * sr, 4 * sr + 1,
-                    8 * sr - 1, 8 * sr, 8 * sr + 1, 13 * sr + 5})
+    sizes = {17, 33}
+    for sr in _STEP[prec]:
+        sizes |= {sr - 1, sr, sr + 1, 2 * sr + 3, 4 * sr - 1, 4 * sr, 4 * sr + 1,
+                  8 * sr - 1, 8 * sr, 8 * sr + 1, 13 * sr + 5}
+    sizes = sorted(sizes)
     for n in sizes:
         for batch in (1, 9, 130):
             announce("thomas steps", prec, n, batch)
@@ -82,4 +84,24 @@ def test_thomas_leaves_c_prime_in_c(prec):
     for i in range(1, n):
         cp[:, i] = C[:, i] / (B[:, i] - A[:, i] * cp[:, i - 1])
     err = np.max(np.abs(cg[:, :-1] - cp[:, :-1])) / np.max(np.abs(cp[:, :-1]))
+    assert err <= TOL[DT[prec]], err
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_thomas_row_scaling(prec):
+    # rows scaled by powers of two spread over 2^-40..2^40 (FP32) / 2^-300..2^300
+    # (FP64): the solution does not change, while products of a few rows'
+    # coefficients leave the format's range unless the composed maps are rescaled
+    n, batch = 3000, 17
+    announce("thomas row scaling", prec, n, batch)
+    a, b, c, d = make_case(n, batch, 0, DT[prec], seed=41)
+    rng = np.random.default_rng(9)
+    span = 40 if prec == 0 else 300
+    s = np.ldexp(1.0, rng.integers(-span, span + 1, size=n * batch)).astype(DT[prec])
+    arrs = [v * s for v in (a, b, c, d)]
+    x = run_solve(arrs, n, batch, 0, 1, DT[prec])
+    xs = run_solve([a, b, c, d], n, batch, 0, 1, DT[prec])
+    st, res = check_against_oracle([a, b, c, d], x, n, batch, 0, DT[prec])
+    assert st["nonfinite"] == 0 and st["relerr"] <= TOL[DT[prec]], st
+    err = np.max(np.abs(x.astype(np.float64) - xs)) / np.max(np.abs(xs))
     assert err <= TOL[DT[prec]], err

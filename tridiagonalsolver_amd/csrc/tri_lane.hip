@@ -8,6 +8,8 @@
 // memory), so a solve moves 5 words per unknown.  For systems of up to N rows:
 // the contiguous layout reads each lane's rows as 16-byte pieces, the interleaved
 // layout reads one row of 64 consecutive systems per load instruction.
+#include <stdlib.h>
+
 #include "tri_host.hpp"
 
 namespace tri {
@@ -319,6 +321,241 @@ __global__ void __launch_bounds__(64) thomas_team_kernel(const T *a, const T *b,
     }
 }
 
+// ---------------------------------------------------------------- Thomas by wave scans
+// The same recurrence and memory semantics (c' into c and d' into d on the way
+// down, x into d on the way up: 9 words per unknown), one system per wave, with
+// the serial chain evaluated by parallel prefix over the wave's 64 lanes.  A step
+// covers SR = 64 * 16/w rows; lane t holds the t-th 16-byte vector (VV rows) of
+// each array.
+//   c'_j = c_j / (b_j - a_j c'_{j-1}) is the linear-fractional map with matrix
+//     [[0, c_j], [-a_j, b_j]] applied to c'_{j-1}.  Each lane composes the maps of
+//     its VV rows; an exclusive scan of the composed 2x2 matrices over the lanes
+//     gives every lane c' of the row before its vector from the step's incoming
+//     c' with one division.  The lane then runs the recurrence over its own VV
+//     rows, which gives each row's pivot t_j = b_j - a_j c'_{j-1} and 1/t_j.
+//     The matrices are scaled by powers of two (exact) after every product:
+//     the map does not change, and no entry overflows.
+//   d'_j = (d_j - a_j d'_{j-1}) / t_j is affine in d'_{j-1}; x_j = d'_j - c'_j x_{j+1}
+//     is affine in x_{j+1}.  Both take the same exclusive scan of composed maps
+//     (forward, then backward over the lanes).
+// The dependence carried from one step to the next is the step's last c' and d'
+// (and, going back, its first x): a few operations per SR rows instead of a
+// division chain per row.  Rounding differs from the row-by-row recurrence; the
+// results agree with it to the solver's tolerance.
+template <typename T> struct Mob {
+    T m00, m01, m10, m11;   // c_out = (m00 c_in + m01) / (m10 c_in + m11)
+};
+
+template <typename T> __device__ __forceinline__ T wave_shfl_up(T v, int d) {
+    return __shfl_up(v, (unsigned)d, 64);
+}
+template <typename T> __device__ __forceinline__ T wave_shfl_down(T v, int d) {
+    return __shfl_down(v, (unsigned)d, 64);
+}
+
+__device__ __forceinline__ int frexp_exp(float x) { return __builtin_amdgcn_frexp_expf(x); }
+__device__ __forceinline__ int frexp_exp(double x) { return __builtin_amdgcn_frexp_exp(x); }
+__device__ __forceinline__ float ldexp_t(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
+__device__ __forceinline__ double ldexp_t(double x, int e) { return __builtin_amdgcn_ldexp(x, e); }
+
+// B applied after A, scaled so that the largest entry lies in [0.5, 1).
+template <typename T>
+__device__ __forceinline__ Mob<T> mob_after(const Mob<T> &B, const Mob<T> &A) {
+    Mob<T> r{B.m00 * A.m00 + B.m01 * A.m10, B.m00 * A.m01 + B.m01 * A.m11,
+             B.m10 * A.m00 + B.m11 * A.m10, B.m10 * A.m01 + B.m11 * A.m11};
+    const T mx = fmax(fmax(fabs(r.m00), fabs(r.m01)), fmax(fabs(r.m10), fabs(r.m11)));
+    const int e = __builtin_isfinite(mx) ? frexp_exp(mx) : 0;   // 0 for mx == 0
+    r.m00 = ldexp_t(r.m00, -e);
+    r.m01 = ldexp_t(r.m01, -e);
+    r.m10 = ldexp_t(r.m10, -e);
+    r.m11 = ldexp_t(r.m11, -e);
+    return r;
+}
+
+// Affine map y = p + q x.  B applied after A.
+template <typename T> struct Aff {
+    T p, q;
+};
+template <typename T> __device__ __forceinline__ Aff<T> aff_after(const Aff<T> &B, const Aff<T> &A) {
+    return {B.p + B.q * A.p, B.q * A.q};
+}
+
+// Inclusive scans over the wave: lane t ends with X_t o ... o X_0 (up) or
+// X_t o ... o X_63 (down, for maps applied from the last lane to the first).
+template <typename T> __device__ __forceinline__ Mob<T> mob_scan_up(Mob<T> x, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const Mob<T> y{wave_shfl_up(x.m00, d), wave_shfl_up(x.m01, d), wave_shfl_up(x.m10, d),
+                       wave_shfl_up(x.m11, d)};
+        if (lane >= d) x = mob_after(x, y);
+    }
+    return x;
+}
+template <typename T> __device__ __forceinline__ Aff<T> aff_scan_up(Aff<T> x, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const Aff<T> y{wave_shfl_up(x.p, d), wave_shfl_up(x.q, d)};
+        if (lane >= d) x = aff_after(x, y);
+    }
+    return x;
+}
+template <typename T> __device__ __forceinline__ Aff<T> aff_scan_down(Aff<T> x, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const Aff<T> y{wave_shfl_down(x.p, d), wave_shfl_down(x.q, d)};
+        if (lane + d < 64) x = aff_after(x, y);
+    }
+    return x;
+}
+
+template <typename T, int DF, int DB>
+__global__ void __launch_bounds__(256) thomas_scan_kernel(const T *a, const T *b, T *c, T *d,
+                                                          int64_t n, int64_t batch, int64_t sstride,
+                                                          int32_t bb) {
+    constexpr int VV = 16 / (int)sizeof(T);   // rows per 16-byte vector
+    constexpr int SR = 64 * VV;                // rows per step
+    const int lane = threadIdx.x & 63;
+    const int64_t s = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (s >= batch) return;                    // whole waves; no workgroup barriers below
+    const int64_t base = s * sstride;
+    const __amdgpu_buffer_rsrc_t rr[4] = {make_rsrc(a, bb), make_rsrc(b, bb), make_rsrc(c, bb),
+                                          make_rsrc(d, bb)};
+    const int64_t steps = (n + SR - 1) / SR;
+    const int64_t own = (int64_t)lane * VV;
+    // unguarded vector loads, as in thomas_team_kernel: sstride is a multiple of VV
+    // and at least n; vectors outside the system read the array's first vector
+    auto vload = [&](int r, int64_t e) -> u32x4 {
+        const int off = ((uint64_t)e < (uint64_t)n) ? (int)((base + e) * (int64_t)sizeof(T)) : 0;
+        return __builtin_amdgcn_raw_buffer_load_b128(rr[r], off, 0, cpol<false>());
+    };
+
+    // ---- forward: c' and d'
+    u32x4 ring[DF][4];
+#pragma unroll
+    for (int k = 0; k < DF; ++k) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ring[k][r] = vload(r, k * SR + own);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    T cp_in = 0, dp_in = 0;                    // c', d' of the row before the step
+    for (int64_t q0 = 0; q0 < steps; q0 += DF) {
+#pragma unroll
+        for (int k = 0; k < DF; ++k) {
+            const int64_t q = q0 + k;
+            const int64_t r0 = q * SR + own;   // first row of the lane's vector
+            u32x4 cur[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                cur[r] = ring[k][r];
+                ring[k][r] = vload(r, (q + DF) * SR + own);
+            }
+            if (q >= steps) continue;          // wave-uniform
+            T A[VV], B[VV], C[VV], D[VV];
+#pragma unroll
+            for (int v = 0; v < VV; ++v) {
+                const int64_t i = r0 + v;
+                const bool in = i < n;
+                A[v] = (in && i > 0) ? reinterpret_cast<const T *>(&cur[0])[v] : T(0);
+                B[v] = in ? reinterpret_cast<const T *>(&cur[1])[v] : T(1);
+                C[v] = (in && i < n - 1) ? reinterpret_cast<const T *>(&cur[2])[v] : T(0);
+                D[v] = in ? reinterpret_cast<const T *>(&cur[3])[v] : T(0);
+            }
+            Mob<T> P{T(0), C[0], -A[0], B[0]};
+#pragma unroll
+            for (int v = 1; v < VV; ++v) P = mob_after(Mob<T>{T(0), C[v], -A[v], B[v]}, P);
+            const Mob<T> I = mob_scan_up(P, lane);
+            Mob<T> E{wave_shfl_up(I.m00, 1), wave_shfl_up(I.m01, 1), wave_shfl_up(I.m10, 1),
+                     wave_shfl_up(I.m11, 1)};
+            if (lane == 0) E = Mob<T>{T(1), T(0), T(0), T(1)};
+            T cprev = (E.m00 * cp_in + E.m01) * rcp(E.m10 * cp_in + E.m11);
+            T al[VV], ga[VV];
+            u32x4 oc;
+#pragma unroll
+            for (int v = 0; v < VV; ++v) {
+                const T it = rcp(B[v] - A[v] * cprev);
+                cprev = C[v] * it;
+                reinterpret_cast<T *>(&oc)[v] = cprev;
+                al[v] = D[v] * it;
+                ga[v] = -(A[v] * it);
+            }
+            Aff<T> L{al[0], ga[0]};
+#pragma unroll
+            for (int v = 1; v < VV; ++v) L = aff_after(Aff<T>{al[v], ga[v]}, L);
+            const Aff<T> J = aff_scan_up(L, lane);
+            Aff<T> F{wave_shfl_up(J.p, 1), wave_shfl_up(J.q, 1)};
+            if (lane == 0) F = Aff<T>{T(0), T(1)};
+            T dprev = F.p + F.q * dp_in;
+            u32x4 od;
+#pragma unroll
+            for (int v = 0; v < VV; ++v) {
+                dprev = al[v] + ga[v] * dprev;
+                reinterpret_cast<T *>(&od)[v] = dprev;
+            }
+            sys_chunk_store<T>(oc, rr[2], base, r0, n);
+            sys_chunk_store<T>(od, rr[3], base, r0, n);
+            cp_in = __shfl(cprev, 63, 64);
+            dp_in = __shfl(dprev, 63, 64);
+        }
+    }
+
+    // ---- backward: x_j = d'_j - c'_j x_{j+1}, last step first (each lane reads back
+    // the vectors it stored)
+    u32x4 back[DB][2];
+#pragma unroll
+    for (int k = 0; k < DB; ++k) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) back[k][r] = vload(2 + r, (steps - 1 - k) * SR + own);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    T x_in = 0;                                // x of the row after the step
+    for (int64_t q0 = steps - 1; q0 >= 0; q0 -= DB) {
+#pragma unroll
+        for (int k = 0; k < DB; ++k) {
+            const int64_t q = q0 - k;
+            const int64_t r0 = q * SR + own;
+            u32x4 cur[2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                cur[r] = back[k][r];
+                back[k][r] = vload(2 + r, (q - DB) * SR + own);
+            }
+            if (q < 0) continue;
+            T Cp[VV], Dp[VV];
+#pragma unroll
+            for (int v = 0; v < VV; ++v) {
+                const bool in = r0 + v < n;
+                Cp[v] = in ? reinterpret_cast<const T *>(&cur[0])[v] : T(0);
+                Dp[v] = in ? reinterpret_cast<const T *>(&cur[1])[v] : T(0);
+            }
+            // the lane's map from x after its vector to x of its first row
+            Aff<T> L{Dp[VV - 1], -Cp[VV - 1]};
+#pragma unroll
+            for (int v = VV - 2; v >= 0; --v) L = aff_after(Aff<T>{Dp[v], -Cp[v]}, L);
+            const Aff<T> J = aff_scan_down(L, lane);
+            Aff<T> F{wave_shfl_down(J.p, 1), wave_shfl_down(J.q, 1)};
+            if (lane == 63) F = Aff<T>{T(0), T(1)};
+            T xn = F.p + F.q * x_in;
+            u32x4 ox;
+#pragma unroll
+            for (int v = VV - 1; v >= 0; --v) {
+                xn = Dp[v] - Cp[v] * xn;
+                reinterpret_cast<T *>(&ox)[v] = xn;
+            }
+            sys_chunk_store<T>(ox, rr[3], base, r0, n);
+            x_in = __shfl(xn, 0, 64);
+        }
+    }
+}
+
+// $TRIDIAG_THOMAS_SCAN=0 keeps the 16-lane team kernel.
+static bool thomas_scan_enabled() {
+    static const bool v = [] {
+        const char *e = getenv("TRIDIAG_THOMAS_SCAN");
+        return !(e && e[0] == '0');
+    }();
+    return v;
+}
+
 template <typename T>
 static hipError_t launch_thomas_rows(const T *a, const T *b, T *c, T *d, int64_t n, int64_t batch,
                                      int64_t sstride, int32_t bb, hipStream_t st) {
@@ -332,6 +569,13 @@ static hipError_t launch_thomas_rows(const T *a, const T *b, T *c, T *d, int64_t
     if (batch <= 0) return hipSuccess;
     const int64_t blocks = (batch + 64 / L - 1) / (64 / L);
     if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+    if (thomas_scan_enabled()) {
+        const int64_t wg = (batch + 3) / 4;
+        if (wg > 0x7fffffff) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((thomas_scan_kernel<T, 4, 8>), dim3((unsigned)wg), dim3(256), 0, st, a,
+                           b, c, d, n, batch, sstride, bb);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL((thomas_team_kernel<T, L, 4, 8>), dim3((unsigned)blocks), dim3(64), 0, st,
                        a, b, c, d, n, batch, sstride, bb);
     return hipGetLastError();
