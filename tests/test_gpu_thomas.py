@@ -46,9 +46,10 @@ def test_thomas_low_margin_long(prec):
     assert st["nonfinite"] == 0 and st["relerr"] <= TOL[DT[prec]], st
 
 
-# rows per step: 64 lanes x one 16-byte vector (the scan kernel) and 16 lanes x one
-# vector (the team kernel); prefetch rings of 4 steps (forward) and 8 steps (backward)
-_STEP = {0: (256, 64), 1: (128, 32)}
+# rows per step: 64 lanes x one or two 16-byte vectors (the scan kernel) and 16 lanes x
+# one vector (the team kernel); prefetch rings of 2-4 steps (forward) and 4-8 steps
+# (backward)
+_STEP = {0: (512, 256, 64), 1: (256, 128, 32)}
 
 
 @pytest.mark.parametrize("prec", [0, 1])
@@ -105,3 +106,14 @@ def test_thomas_row_scaling(prec):
     assert st["nonfinite"] == 0 and st["relerr"] <= TOL[DT[prec]], st
     err = np.max(np.abs(x.astype(np.float64) - xs)) / np.max(np.abs(xs))
     assert err <= TOL[DT[prec]], err
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_thomas_both_step_widths(prec):
+    # batches up to 2048 take two 16-byte vectors per lane and step, larger ones one
+    for n, batch in ((1000, 2048), (1000, 2049), (4099, 2050), (257, 3000)):
+        announce("thomas step width", prec, n, batch)
+        arrs = make_case(n, batch, 0, DT[prec], seed=n + batch)
+        x = run_solve(arrs, n, batch, 0, 1, DT[prec])
+        st, res = check_against_oracle(arrs, x, n, batch, 0, DT[prec])
+        assert st["nonfinite"] == 0 and st["relerr"] <= TOL[DT[prec]], (n, batch, st)

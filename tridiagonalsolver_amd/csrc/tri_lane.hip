@@ -346,13 +346,6 @@ template <typename T> struct Mob {
     T m00, m01, m10, m11;   // c_out = (m00 c_in + m01) / (m10 c_in + m11)
 };
 
-template <typename T> __device__ __forceinline__ T wave_shfl_up(T v, int d) {
-    return __shfl_up(v, (unsigned)d, 64);
-}
-template <typename T> __device__ __forceinline__ T wave_shfl_down(T v, int d) {
-    return __shfl_down(v, (unsigned)d, 64);
-}
-
 __device__ __forceinline__ int frexp_exp(float x) { return __builtin_amdgcn_frexp_expf(x); }
 __device__ __forceinline__ int frexp_exp(double x) { return __builtin_amdgcn_frexp_exp(x); }
 __device__ __forceinline__ float ldexp_t(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
@@ -380,40 +373,141 @@ template <typename T> __device__ __forceinline__ Aff<T> aff_after(const Aff<T> &
     return {B.p + B.q * A.p, B.q * A.q};
 }
 
-// Inclusive scans over the wave: lane t ends with X_t o ... o X_0 (up) or
-// X_t o ... o X_63 (down, for maps applied from the last lane to the first).
-template <typename T> __device__ __forceinline__ Mob<T> mob_scan_up(Mob<T> x, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const Mob<T> y{wave_shfl_up(x.m00, d), wave_shfl_up(x.m01, d), wave_shfl_up(x.m10, d),
-                       wave_shfl_up(x.m11, d)};
-        if (lane >= d) x = mob_after(x, y);
-    }
-    return x;
+// Moves between lanes.  DPP (data-parallel primitives: the source lane is picked by
+// a modifier on the VALU move, a few cycles) or ds_bpermute (LDS crossbar, tens of
+// cycles).  dpp_mov<CTRL, ROWS>(old, x): x of the lane that the DPP control CTRL
+// names, or old where that lane lies outside the row or the wave, and in rows not
+// in the mask ROWS.
+template <int CTRL, int ROWS> __device__ __forceinline__ float dpp_mov(float old, float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                 __builtin_bit_cast(int, x), CTRL,
+                                                                 ROWS, 0xf, false));
 }
-template <typename T> __device__ __forceinline__ Aff<T> aff_scan_up(Aff<T> x, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const Aff<T> y{wave_shfl_up(x.p, d), wave_shfl_up(x.q, d)};
-        if (lane >= d) x = aff_after(x, y);
-    }
-    return x;
+template <int CTRL, int ROWS> __device__ __forceinline__ double dpp_mov(double old, double x) {
+    const uint64_t o = __builtin_bit_cast(uint64_t, old), v = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)o, (int)(uint32_t)v,
+                                                              CTRL, ROWS, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(
+        (int)(uint32_t)(o >> 32), (int)(uint32_t)(v >> 32), CTRL, ROWS, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
-template <typename T> __device__ __forceinline__ Aff<T> aff_scan_down(Aff<T> x, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const Aff<T> y{wave_shfl_down(x.p, d), wave_shfl_down(x.q, d)};
-        if (lane + d < 64) x = aff_after(x, y);
-    }
-    return x;
+__device__ __forceinline__ float read_lane(float x, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+}
+__device__ __forceinline__ double read_lane(double x, int l) {
+    const uint64_t v = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+// DPP controls of GFX9: row_shr:d 0x110+d, row_shl:d 0x100+d (inside rows of 16
+// lanes), wave_shr:1 0x138, wave_shl:1 0x130, row_bcast:15 0x142 (lane 15 of each
+// row into the next row), row_bcast:31 0x143 (lane 31 into rows 2 and 3).
+template <int CTRL, int ROWS, typename T> __device__ __forceinline__ Mob<T> mob_dpp(const Mob<T> &x) {
+    return {dpp_mov<CTRL, ROWS>(T(1), x.m00), dpp_mov<CTRL, ROWS>(T(0), x.m01),
+            dpp_mov<CTRL, ROWS>(T(0), x.m10), dpp_mov<CTRL, ROWS>(T(1), x.m11)};
+}
+template <int CTRL, int ROWS, typename T> __device__ __forceinline__ Aff<T> aff_dpp(const Aff<T> &x) {
+    return {dpp_mov<CTRL, ROWS>(T(0), x.p), dpp_mov<CTRL, ROWS>(T(1), x.q)};
 }
 
-template <typename T, int DF, int DB>
+// Scans over the wave.  Inclusive up: lane t ends with X_t o ... o X_0; exclusive
+// up: X_(t-1) o ... o X_0 (identity at lane 0).  Inclusive down: X_t o ... o X_63
+// (maps applied from the last lane to the first); exclusive down: X_(t+1) o ... o
+// X_63 (identity at lane 63).  Lanes without a partner compose with the identity,
+// so the scans have no lane-dependent branches.
+template <bool DPP> struct WaveScan;
+template <> struct WaveScan<true> {
+    template <typename T> __device__ static Mob<T> up(Mob<T> x, int) {
+        x = mob_after(x, mob_dpp<0x111, 0xf>(x));
+        x = mob_after(x, mob_dpp<0x112, 0xf>(x));
+        x = mob_after(x, mob_dpp<0x114, 0xf>(x));
+        x = mob_after(x, mob_dpp<0x118, 0xf>(x));
+        x = mob_after(x, mob_dpp<0x142, 0xa>(x));
+        return mob_after(x, mob_dpp<0x143, 0xc>(x));
+    }
+    template <typename T> __device__ static Mob<T> excl_up(const Mob<T> &x, int) {
+        return mob_dpp<0x138, 0xf>(x);
+    }
+    template <typename T> __device__ static Aff<T> up(Aff<T> x, int) {
+        x = aff_after(x, aff_dpp<0x111, 0xf>(x));
+        x = aff_after(x, aff_dpp<0x112, 0xf>(x));
+        x = aff_after(x, aff_dpp<0x114, 0xf>(x));
+        x = aff_after(x, aff_dpp<0x118, 0xf>(x));
+        x = aff_after(x, aff_dpp<0x142, 0xa>(x));
+        return aff_after(x, aff_dpp<0x143, 0xc>(x));
+    }
+    template <typename T> __device__ static Aff<T> excl_up(const Aff<T> &x, int) {
+        return aff_dpp<0x138, 0xf>(x);
+    }
+    template <typename T> __device__ static Aff<T> down(Aff<T> x, int lane) {
+        x = aff_after(x, aff_dpp<0x101, 0xf>(x));
+        x = aff_after(x, aff_dpp<0x102, 0xf>(x));
+        x = aff_after(x, aff_dpp<0x104, 0xf>(x));
+        x = aff_after(x, aff_dpp<0x108, 0xf>(x));
+        // lane 16r now holds row r's composite; rows below take those of the rows above
+        const Aff<T> t3{read_lane(x.p, 48), read_lane(x.q, 48)};
+        const Aff<T> t2 = aff_after(Aff<T>{read_lane(x.p, 32), read_lane(x.q, 32)}, t3);
+        const Aff<T> t1 = aff_after(Aff<T>{read_lane(x.p, 16), read_lane(x.q, 16)}, t2);
+        const int row = lane >> 4;
+        const Aff<T> y = row == 0 ? t1 : row == 1 ? t2 : row == 2 ? t3 : Aff<T>{T(0), T(1)};
+        return aff_after(x, y);
+    }
+    template <typename T> __device__ static Aff<T> excl_down(const Aff<T> &x, int) {
+        return aff_dpp<0x130, 0xf>(x);
+    }
+    template <typename T> __device__ static T at(T x, int l) { return read_lane(x, l); }
+};
+template <> struct WaveScan<false> {
+    template <typename T> __device__ static T sh_up(T v, int d) { return __shfl_up(v, (unsigned)d, 64); }
+    template <typename T> __device__ static T sh_dn(T v, int d) { return __shfl_down(v, (unsigned)d, 64); }
+    template <typename T> __device__ static Mob<T> up(Mob<T> x, int lane) {
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const Mob<T> y{sh_up(x.m00, d), sh_up(x.m01, d), sh_up(x.m10, d), sh_up(x.m11, d)};
+            if (lane >= d) x = mob_after(x, y);
+        }
+        return x;
+    }
+    template <typename T> __device__ static Mob<T> excl_up(const Mob<T> &x, int lane) {
+        const Mob<T> y{sh_up(x.m00, 1), sh_up(x.m01, 1), sh_up(x.m10, 1), sh_up(x.m11, 1)};
+        return lane == 0 ? Mob<T>{T(1), T(0), T(0), T(1)} : y;
+    }
+    template <typename T> __device__ static Aff<T> up(Aff<T> x, int lane) {
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const Aff<T> y{sh_up(x.p, d), sh_up(x.q, d)};
+            if (lane >= d) x = aff_after(x, y);
+        }
+        return x;
+    }
+    template <typename T> __device__ static Aff<T> excl_up(const Aff<T> &x, int lane) {
+        const Aff<T> y{sh_up(x.p, 1), sh_up(x.q, 1)};
+        return lane == 0 ? Aff<T>{T(0), T(1)} : y;
+    }
+    template <typename T> __device__ static Aff<T> down(Aff<T> x, int lane) {
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const Aff<T> y{sh_dn(x.p, d), sh_dn(x.q, d)};
+            if (lane + d < 64) x = aff_after(x, y);
+        }
+        return x;
+    }
+    template <typename T> __device__ static Aff<T> excl_down(const Aff<T> &x, int lane) {
+        const Aff<T> y{sh_dn(x.p, 1), sh_dn(x.q, 1)};
+        return lane == 63 ? Aff<T>{T(0), T(1)} : y;
+    }
+    template <typename T> __device__ static T at(T x, int l) { return __shfl(x, l, 64); }
+};
+
+template <typename T, int NV, int DF, int DB, bool DPP>
 __global__ void __launch_bounds__(256) thomas_scan_kernel(const T *a, const T *b, T *c, T *d,
                                                           int64_t n, int64_t batch, int64_t sstride,
                                                           int32_t bb) {
     constexpr int VV = 16 / (int)sizeof(T);   // rows per 16-byte vector
-    constexpr int SR = 64 * VV;                // rows per step
+    constexpr int R = NV * VV;                 // rows per lane and step
+    constexpr int SR = 64 * R;                 // rows per step
+    using W = WaveScan<DPP>;
     const int lane = threadIdx.x & 63;
     const int64_t s = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (s >= batch) return;                    // whole waves; no workgroup barriers below
@@ -421,7 +515,7 @@ __global__ void __launch_bounds__(256) thomas_scan_kernel(const T *a, const T *b
     const __amdgpu_buffer_rsrc_t rr[4] = {make_rsrc(a, bb), make_rsrc(b, bb), make_rsrc(c, bb),
                                           make_rsrc(d, bb)};
     const int64_t steps = (n + SR - 1) / SR;
-    const int64_t own = (int64_t)lane * VV;
+    const int64_t own = (int64_t)lane * R;
     // unguarded vector loads, as in thomas_team_kernel: sstride is a multiple of VV
     // and at least n; vectors outside the system read the array's first vector
     auto vload = [&](int r, int64_t e) -> u32x4 {
@@ -430,11 +524,13 @@ __global__ void __launch_bounds__(256) thomas_scan_kernel(const T *a, const T *b
     };
 
     // ---- forward: c' and d'
-    u32x4 ring[DF][4];
+    u32x4 ring[DF][4][NV];
 #pragma unroll
     for (int k = 0; k < DF; ++k) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) ring[k][r] = vload(r, k * SR + own);
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) ring[k][r][v] = vload(r, k * SR + own + v * VV);
         __builtin_amdgcn_sched_barrier(0);
     }
     T cp_in = 0, dp_in = 0;                    // c', d' of the row before the step
@@ -442,69 +538,73 @@ __global__ void __launch_bounds__(256) thomas_scan_kernel(const T *a, const T *b
 #pragma unroll
         for (int k = 0; k < DF; ++k) {
             const int64_t q = q0 + k;
-            const int64_t r0 = q * SR + own;   // first row of the lane's vector
-            u32x4 cur[4];
+            const int64_t r0 = q * SR + own;   // first row of the lane's rows
+            u32x4 cur[4][NV];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                cur[r] = ring[k][r];
-                ring[k][r] = vload(r, (q + DF) * SR + own);
-            }
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    cur[r][v] = ring[k][r][v];
+                    ring[k][r][v] = vload(r, (q + DF) * SR + own + v * VV);
+                }
             if (q >= steps) continue;          // wave-uniform
-            T A[VV], B[VV], C[VV], D[VV];
+            T A[R], B[R], C[R], D[R];
 #pragma unroll
-            for (int v = 0; v < VV; ++v) {
-                const int64_t i = r0 + v;
+            for (int j = 0; j < R; ++j) {
+                const int64_t i = r0 + j;
                 const bool in = i < n;
-                A[v] = (in && i > 0) ? reinterpret_cast<const T *>(&cur[0])[v] : T(0);
-                B[v] = in ? reinterpret_cast<const T *>(&cur[1])[v] : T(1);
-                C[v] = (in && i < n - 1) ? reinterpret_cast<const T *>(&cur[2])[v] : T(0);
-                D[v] = in ? reinterpret_cast<const T *>(&cur[3])[v] : T(0);
+                const int v = j / VV, e = j % VV;
+                A[j] = (in && i > 0) ? reinterpret_cast<const T *>(&cur[0][v])[e] : T(0);
+                B[j] = in ? reinterpret_cast<const T *>(&cur[1][v])[e] : T(1);
+                C[j] = (in && i < n - 1) ? reinterpret_cast<const T *>(&cur[2][v])[e] : T(0);
+                D[j] = in ? reinterpret_cast<const T *>(&cur[3][v])[e] : T(0);
             }
             Mob<T> P{T(0), C[0], -A[0], B[0]};
 #pragma unroll
-            for (int v = 1; v < VV; ++v) P = mob_after(Mob<T>{T(0), C[v], -A[v], B[v]}, P);
-            const Mob<T> I = mob_scan_up(P, lane);
-            Mob<T> E{wave_shfl_up(I.m00, 1), wave_shfl_up(I.m01, 1), wave_shfl_up(I.m10, 1),
-                     wave_shfl_up(I.m11, 1)};
-            if (lane == 0) E = Mob<T>{T(1), T(0), T(0), T(1)};
+            for (int j = 1; j < R; ++j) P = mob_after(Mob<T>{T(0), C[j], -A[j], B[j]}, P);
+            const Mob<T> E = W::excl_up(W::up(P, lane), lane);
             T cprev = (E.m00 * cp_in + E.m01) * rcp(E.m10 * cp_in + E.m11);
-            T al[VV], ga[VV];
-            u32x4 oc;
+            T al[R], ga[R];
+            u32x4 oc[NV];
 #pragma unroll
-            for (int v = 0; v < VV; ++v) {
-                const T it = rcp(B[v] - A[v] * cprev);
-                cprev = C[v] * it;
-                reinterpret_cast<T *>(&oc)[v] = cprev;
-                al[v] = D[v] * it;
-                ga[v] = -(A[v] * it);
+            for (int j = 0; j < R; ++j) {
+                const T it = rcp(B[j] - A[j] * cprev);
+                cprev = C[j] * it;
+                reinterpret_cast<T *>(&oc[j / VV])[j % VV] = cprev;
+                al[j] = D[j] * it;
+                ga[j] = -(A[j] * it);
             }
             Aff<T> L{al[0], ga[0]};
 #pragma unroll
-            for (int v = 1; v < VV; ++v) L = aff_after(Aff<T>{al[v], ga[v]}, L);
-            const Aff<T> J = aff_scan_up(L, lane);
-            Aff<T> F{wave_shfl_up(J.p, 1), wave_shfl_up(J.q, 1)};
-            if (lane == 0) F = Aff<T>{T(0), T(1)};
+            for (int j = 1; j < R; ++j) L = aff_after(Aff<T>{al[j], ga[j]}, L);
+            const Aff<T> F = W::excl_up(W::up(L, lane), lane);
             T dprev = F.p + F.q * dp_in;
-            u32x4 od;
+            u32x4 od[NV];
 #pragma unroll
-            for (int v = 0; v < VV; ++v) {
-                dprev = al[v] + ga[v] * dprev;
-                reinterpret_cast<T *>(&od)[v] = dprev;
+            for (int j = 0; j < R; ++j) {
+                dprev = al[j] + ga[j] * dprev;
+                reinterpret_cast<T *>(&od[j / VV])[j % VV] = dprev;
             }
-            sys_chunk_store<T>(oc, rr[2], base, r0, n);
-            sys_chunk_store<T>(od, rr[3], base, r0, n);
-            cp_in = __shfl(cprev, 63, 64);
-            dp_in = __shfl(dprev, 63, 64);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                sys_chunk_store<T>(oc[v], rr[2], base, r0 + v * VV, n);
+                sys_chunk_store<T>(od[v], rr[3], base, r0 + v * VV, n);
+            }
+            cp_in = W::at(cprev, 63);
+            dp_in = W::at(dprev, 63);
         }
     }
 
     // ---- backward: x_j = d'_j - c'_j x_{j+1}, last step first (each lane reads back
     // the vectors it stored)
-    u32x4 back[DB][2];
+    u32x4 back[DB][2][NV];
 #pragma unroll
     for (int k = 0; k < DB; ++k) {
 #pragma unroll
-        for (int r = 0; r < 2; ++r) back[k][r] = vload(2 + r, (steps - 1 - k) * SR + own);
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                back[k][r][v] = vload(2 + r, (steps - 1 - k) * SR + own + v * VV);
         __builtin_amdgcn_sched_barrier(0);
     }
     T x_in = 0;                                // x of the row after the step
@@ -513,47 +613,62 @@ __global__ void __launch_bounds__(256) thomas_scan_kernel(const T *a, const T *b
         for (int k = 0; k < DB; ++k) {
             const int64_t q = q0 - k;
             const int64_t r0 = q * SR + own;
-            u32x4 cur[2];
+            u32x4 cur[2][NV];
 #pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                cur[r] = back[k][r];
-                back[k][r] = vload(2 + r, (q - DB) * SR + own);
-            }
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    cur[r][v] = back[k][r][v];
+                    back[k][r][v] = vload(2 + r, (q - DB) * SR + own + v * VV);
+                }
             if (q < 0) continue;
-            T Cp[VV], Dp[VV];
+            T Cp[R], Dp[R];
 #pragma unroll
-            for (int v = 0; v < VV; ++v) {
-                const bool in = r0 + v < n;
-                Cp[v] = in ? reinterpret_cast<const T *>(&cur[0])[v] : T(0);
-                Dp[v] = in ? reinterpret_cast<const T *>(&cur[1])[v] : T(0);
+            for (int j = 0; j < R; ++j) {
+                const bool in = r0 + j < n;
+                Cp[j] = in ? reinterpret_cast<const T *>(&cur[0][j / VV])[j % VV] : T(0);
+                Dp[j] = in ? reinterpret_cast<const T *>(&cur[1][j / VV])[j % VV] : T(0);
             }
-            // the lane's map from x after its vector to x of its first row
-            Aff<T> L{Dp[VV - 1], -Cp[VV - 1]};
+            // the lane's map from x after its rows to x of its first row
+            Aff<T> L{Dp[R - 1], -Cp[R - 1]};
 #pragma unroll
-            for (int v = VV - 2; v >= 0; --v) L = aff_after(Aff<T>{Dp[v], -Cp[v]}, L);
-            const Aff<T> J = aff_scan_down(L, lane);
-            Aff<T> F{wave_shfl_down(J.p, 1), wave_shfl_down(J.q, 1)};
-            if (lane == 63) F = Aff<T>{T(0), T(1)};
+            for (int j = R - 2; j >= 0; --j) L = aff_after(Aff<T>{Dp[j], -Cp[j]}, L);
+            const Aff<T> F = W::excl_down(W::down(L, lane), lane);
             T xn = F.p + F.q * x_in;
-            u32x4 ox;
+            u32x4 ox[NV];
 #pragma unroll
-            for (int v = VV - 1; v >= 0; --v) {
-                xn = Dp[v] - Cp[v] * xn;
-                reinterpret_cast<T *>(&ox)[v] = xn;
+            for (int j = R - 1; j >= 0; --j) {
+                xn = Dp[j] - Cp[j] * xn;
+                reinterpret_cast<T *>(&ox[j / VV])[j % VV] = xn;
             }
-            sys_chunk_store<T>(ox, rr[3], base, r0, n);
-            x_in = __shfl(xn, 0, 64);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sys_chunk_store<T>(ox[v], rr[3], base, r0 + v * VV, n);
+            x_in = W::at(xn, 0);
         }
     }
 }
 
-// $TRIDIAG_THOMAS_SCAN=0 keeps the 16-lane team kernel.
-static bool thomas_scan_enabled() {
-    static const bool v = [] {
+// $TRIDIAG_THOMAS_SCAN: 0 keeps the 16-lane team kernel, 2 moves the scan kernel's
+// values between lanes through ds_bpermute instead of DPP; $TRIDIAG_THOMAS_NV fixes
+// the 16-byte vectors per lane and step (1 or 2).
+static int thomas_scan_mode() {
+    static const int v = [] {
         const char *e = getenv("TRIDIAG_THOMAS_SCAN");
-        return !(e && e[0] == '0');
+        return (e && (e[0] == '0' || e[0] == '2')) ? e[0] - '0' : 1;
     }();
     return v;
+}
+// Two vectors per lane halve the steps, and with them the scans on the chain: the
+// better choice while the batch leaves most SIMDs without a second wave.  Measured
+// on MI355X (ms, one / two vectors): FP64 1024 x 1024 0.0179 / 0.0168, FP64 512 x
+// 1024 0.0111 / 0.0103, FP64 65536 x 256 0.687 / 0.539; at 4096 x 4096 one vector
+// keeps more loads in flight per register: FP64 0.203 / 0.211, FP32 0.103 / 0.111.
+static int thomas_scan_nv(int64_t batch) {
+    static const int env = [] {
+        const char *e = getenv("TRIDIAG_THOMAS_NV");
+        return (e && (e[0] == '1' || e[0] == '2')) ? e[0] - '0' : 0;
+    }();
+    return env ? env : (batch <= 2048 ? 2 : 1);
 }
 
 template <typename T>
@@ -569,11 +684,19 @@ static hipError_t launch_thomas_rows(const T *a, const T *b, T *c, T *d, int64_t
     if (batch <= 0) return hipSuccess;
     const int64_t blocks = (batch + 64 / L - 1) / (64 / L);
     if (blocks > 0x7fffffff) return hipErrorInvalidValue;
-    if (thomas_scan_enabled()) {
+    if (thomas_scan_mode() != 0) {
         const int64_t wg = (batch + 3) / 4;
         if (wg > 0x7fffffff) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((thomas_scan_kernel<T, 4, 8>), dim3((unsigned)wg), dim3(256), 0, st, a,
-                           b, c, d, n, batch, sstride, bb);
+        const dim3 grid((unsigned)wg), block(256);
+        if (thomas_scan_mode() == 2)
+            hipLaunchKernelGGL((thomas_scan_kernel<T, 1, 4, 8, false>), grid, block, 0, st, a, b,
+                               c, d, n, batch, sstride, bb);
+        else if (thomas_scan_nv(batch) == 2)
+            hipLaunchKernelGGL((thomas_scan_kernel<T, 2, 2, 4, true>), grid, block, 0, st, a, b, c,
+                               d, n, batch, sstride, bb);
+        else
+            hipLaunchKernelGGL((thomas_scan_kernel<T, 1, 4, 8, true>), grid, block, 0, st, a, b, c,
+                               d, n, batch, sstride, bb);
         return hipGetLastError();
     }
     hipLaunchKernelGGL((thomas_team_kernel<T, L, 4, 8>), dim3((unsigned)blocks), dim3(64), 0, st,
