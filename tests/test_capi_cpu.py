@@ -194,3 +194,34 @@ def test_convert_layout_validation_without_gpu(capi):
     assert conv(0, 1, 0, 8, 8, [p[0], nul, nul, nul], [nul, nul, nul, inside]) == capi.EINVAL
     # an empty batch after valid arguments is a no-op
     assert conv(1, 0, 2, 8, 0, p[:4], p[4:]) == capi.OK
+
+
+@pytest.mark.parametrize("mode", ["refined", "ieee", "fast"])
+def test_division_variants_load(capi, mode):
+    # each division-policy library exports the header's symbols and reports its policy
+    from tridiagonalsolver_amd import build
+    path = os.path.join(os.path.dirname(capi.LIB_PATH), build.lib_name(mode))
+    if not os.path.exists(path):
+        build.build(div_mode=mode)
+    L = ctypes.CDLL(path)
+    for nm in _declared_symbols():
+        assert hasattr(L, nm), (mode, nm)
+    assert L.tridiag_div_mode() == capi.DIV_MODES[mode]
+    assert capi.DIV_MODES[mode] == {"refined": 0, "ieee": 1, "fast": 2}[mode]
+
+
+def test_division_mode_env_names_the_library():
+    import subprocess
+    import sys
+    code = ("from tridiagonalsolver_amd import _capi; import os; "
+            "print(os.path.basename(_capi.LIB_PATH))")
+    for mode, name in (("ieee", "libtridiag_amd_div_ieee.so"),
+                       ("fast", "libtridiag_amd_div_fast.so")):
+        out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True,
+                             text=True, env=dict(os.environ, TRIDIAG_DIV_MODE=mode), check=True)
+        assert out.stdout.strip() == name
+    bad = subprocess.run([sys.executable, "-c",
+                          "from tridiagonalsolver_amd import _capi; _capi.lib()"],
+                         cwd=ROOT, capture_output=True, text=True,
+                         env=dict(os.environ, TRIDIAG_DIV_MODE="approx"))
+    assert bad.returncode != 0 and "TRIDIAG_DIV_MODE" in bad.stderr

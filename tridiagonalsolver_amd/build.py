@@ -1,9 +1,16 @@
 """Builds libtridiag_amd.so (HIP, gfx950) in-tree.
 
     python -m tridiagonalsolver_amd.build          # or: python tridiagonalsolver_amd/build.py
+    python -m tridiagonalsolver_amd.build --div-mode ieee     # one division-policy variant
+    python -m tridiagonalsolver_amd.build --all-div-modes     # the library and both variants
 
 Steps: regenerate the tile-kernel instantiation tables, compile every csrc/*.hip
 to an object with hipcc in parallel, link the shared library into lib/.
+
+Division policy (TRIDIAG_DIV_MODE, fixed per library as the reference fixes its
+division macros per build): "refined" (default, libtridiag_amd.so), "ieee"
+(libtridiag_amd_div_ieee.so) and "fast" (libtridiag_amd_div_fast.so).  The
+variants export the same symbols.
 """
 import concurrent.futures as cf
 import os
@@ -15,6 +22,12 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIBDIR = os.path.join(PKG, "lib")
 LIBNAME = "libtridiag_amd.so"
+DIV_MODES = {"refined": 0, "ieee": 1, "fast": 2}
+
+
+def lib_name(div_mode="refined"):
+    """File name of the library built with a division policy."""
+    return LIBNAME if div_mode == "refined" else "libtridiag_amd_div_%s.so" % div_mode
 ARCH = os.environ.get("TRIDIAG_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -204,9 +217,9 @@ def generate():
 
 
 # ------------------------------------------------------------------ compile
-def _compile(src):
-    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + HIPFLAGS + ["-c", src, "-o", obj]
+def _compile(src, build_dir, defines=()):
+    obj = os.path.join(build_dir, os.path.basename(src) + ".o")
+    cmd = [HIPCC] + HIPFLAGS + list(defines) + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, r.stdout, r.stderr))
@@ -220,18 +233,23 @@ def _headers_mtime():
     return max(os.path.getmtime(h) for h in hs) if hs else 0.0
 
 
-def build(verbose=False, jobs=None, incremental=False):
-    """Compiles every csrc/*.hip and links the library.  incremental=True keeps
+def build(verbose=False, jobs=None, incremental=False, div_mode="refined"):
+    """Compiles every csrc/*.hip and links the library of one division policy
+    (the default one also gets the command-line driver).  incremental=True keeps
     objects newer than their source and every header (a development shortcut; the
     default rebuilds everything)."""
+    if div_mode not in DIV_MODES:
+        raise ValueError("div_mode must be one of %s" % sorted(DIV_MODES))
     generate()
-    os.makedirs(BUILD, exist_ok=True)
+    build_dir = BUILD if div_mode == "refined" else BUILD + "_div_" + div_mode
+    defines = ["-DTRIDIAG_DIV_MODE=%d" % DIV_MODES[div_mode]]
+    os.makedirs(build_dir, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
     hdr_t = _headers_mtime() if incremental else 0.0
     todo, objs = [], []
     for src in srcs:
-        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+        obj = os.path.join(build_dir, os.path.basename(src) + ".o")
         if incremental and os.path.exists(obj) and \
                 os.path.getmtime(obj) > max(os.path.getmtime(src), hdr_t):
             objs.append(obj)
@@ -239,16 +257,18 @@ def build(verbose=False, jobs=None, incremental=False):
             todo.append(src)
     jobs = jobs or min(max(1, len(todo)), max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        for obj, log in ex.map(_compile, todo):
+        for obj, log in ex.map(lambda src: _compile(src, build_dir, defines), todo):
             objs.append(obj)
             if verbose and log.strip():
                 print(log, file=sys.stderr)
-    out = os.path.join(LIBDIR, LIBNAME)
+    out = os.path.join(LIBDIR, lib_name(div_mode))
     tmp = out + ".tmp%d" % os.getpid()
     r = subprocess.run([HIPCC] + objs + LDFLAGS + ["-o", tmp], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, out)
+    if div_mode != "refined":
+        return out
     # the command-line driver (reference test program interface) against the library
     drv = os.path.join(LIBDIR, "tridiag_test")
     r = subprocess.run([HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(os.path.dirname(PKG), "include"),
@@ -261,5 +281,15 @@ def build(verbose=False, jobs=None, incremental=False):
     return out
 
 
+def build_all(verbose=False, jobs=None, incremental=False):
+    """The library and its two division-policy variants."""
+    return [build(verbose, jobs, incremental, m) for m in ("refined", "ieee", "fast")]
+
+
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))
+    inc = "--incremental" in sys.argv
+    if "--all-div-modes" in sys.argv:
+        print("\n".join(build_all(verbose="-v" in sys.argv, incremental=inc)))
+    else:
+        mode = sys.argv[sys.argv.index("--div-mode") + 1] if "--div-mode" in sys.argv else "refined"
+        print(build(verbose="-v" in sys.argv, incremental=inc, div_mode=mode))

@@ -6,19 +6,35 @@
 namespace tri {
 
 // ---------------------------------------------------------------- reciprocal
-// Hardware reciprocal (v_rcp_f32 / v_rcp_f64) refined by Newton-Raphson
-// (y' = y + y(1 - x y)); one step in FP32, two in FP64, which brings the
-// hardware estimate to working precision.  Zero pivots give inf/NaN, which
-// propagate to the solution as they would with IEEE division.
+// Every division of the solvers is a reciprocal followed by products (one pivot
+// reciprocal serves several quotients).  The division policy is fixed when the
+// library is built, as the reference fixes its division macros at compile time
+// (Tridiagonal_CUDA/intrin.h:23-87):
+//   TRIDIAG_DIV_MODE 0 "refined" (default): hardware reciprocal (v_rcp_f32 /
+//     v_rcp_f64) refined by Newton-Raphson, y' = y + y(1 - x y): one step in FP32,
+//     two in FP64, which brings the estimate to working precision;
+//   1 "ieee": the correctly rounded reciprocal, 1/x by IEEE division;
+//   2 "fast": the hardware estimate alone in FP32 (1 ulp), one Newton step in FP64.
+// Zero pivots give inf/NaN in every mode, which propagate to the solution as they
+// would with IEEE division.
+#ifndef TRIDIAG_DIV_MODE
+#define TRIDIAG_DIV_MODE 0
+#endif
+constexpr int kDivMode = TRIDIAG_DIV_MODE;
+static_assert(kDivMode >= 0 && kDivMode <= 2, "TRIDIAG_DIV_MODE must be 0, 1 or 2");
+
 __device__ __forceinline__ float rcp(float x) {
-    float y = __builtin_amdgcn_rcpf(x);
+    if constexpr (kDivMode == 1) return 1.0f / x;
+    const float y = __builtin_amdgcn_rcpf(x);
+    if constexpr (kDivMode == 2) return y;
     return __builtin_fmaf(y, __builtin_fmaf(-x, y, 1.0f), y);
 }
 __device__ __forceinline__ double rcp(double x) {
+    if constexpr (kDivMode == 1) return 1.0 / x;
     double y = __builtin_amdgcn_rcp(x);
     y = __builtin_fma(y, __builtin_fma(-x, y, 1.0), y);
-    y = __builtin_fma(y, __builtin_fma(-x, y, 1.0), y);
-    return y;
+    if constexpr (kDivMode == 2) return y;
+    return __builtin_fma(y, __builtin_fma(-x, y, 1.0), y);
 }
 
 // ---------------------------------------------------------------- cross-lane
