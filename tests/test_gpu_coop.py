@@ -134,3 +134,36 @@ def test_wait_that_gives_up_marks_nan():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "nan_fraction" in r.stdout
+
+
+_FORCED_KERNEL = r"""
+import sys, numpy as np
+sys.path.insert(0, %(root)r)
+sys.path.insert(0, %(root)r + '/tests')
+import tridiagonalsolver_amd as T
+from gpu_util import TOL, check_against_oracle, make_case, run_solve
+for dtype, layout, n, batch in ((np.float32, 2, 4096, 96), (np.float64, 2, 4096, 48),
+                                (np.float32, 2, 8192, 40), (np.float64, 2, 8192, 17),
+                                (np.float32, 0, 20000, 7), (np.float64, 0, 20001, 5),
+                                (np.float32, 0, 65536, 3)):
+    prec = 0 if dtype == np.float32 else 1
+    assert T.plan(prec, -1, layout, n, batch)["kind"] == "cooperative"
+    arrs = make_case(n, batch, layout, dtype, seed=n + batch)
+    x = run_solve(arrs, n, batch, layout, -1, dtype)
+    st, res = check_against_oracle(arrs, x, n, batch, layout, dtype)
+    print(np.dtype(dtype).name, layout, n, batch, st["relerr"], flush=True)
+    assert st["nonfinite"] == 0 and st["relerr"] <= TOL[dtype], st
+print("ok", flush=True)
+"""
+
+
+@pytest.mark.parametrize("pipe", ["0", "1"])
+def test_both_cooperative_kernels(pipe):
+    """TRIDIAG_COOP_PIPE forces the kernel that waits one task later (1) or the one
+    that waits before its next task (0) for every table entry, whichever each
+    entry runs by default (child process: the setting is read once)."""
+    env = dict(os.environ, TRIDIAG_COOP_PIPE=pipe)
+    announce("coop forced kernel (child process)", pipe)
+    r = subprocess.run([sys.executable, "-c", _FORCED_KERNEL % {"root": ROOT}], env=env,
+                       capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout + r.stderr

@@ -60,6 +60,7 @@ template <typename T> struct CoopParams {
     int32_t K;           // row blocks per system
     int32_t ngroups;     // groups of S systems
     int32_t spin_limit;  // polls of a wait before it gives up
+    int32_t pipelined;   // 1: coop_pipe_kernel (wait one task later), 0: coop_kernel
     T *forms;            // [ngroups][K][S][6] block outer forms
     int32_t *ctl;        // [kCoopCtlStride * g] blocks of group g published
 };
@@ -371,6 +372,199 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)
     }
 }
 
+// Back-substitution of one task once its group has published: waits for the K
+// block forms of group gi, chains them to U, V of block j, chains the wave-blocks
+// (blk: this task's wave-block forms, overwritten) and stores x of the block.
+template <typename T, int M, int NW, int S, bool IL, int KMAX, int NI>
+__device__ __forceinline__ void coop_finish(const CoopParams<T> &p, int gi, int j, T (*blk)[NW][6],
+                                            T (*sh_uv)[NW][2], T (*sh_k)[KMAX][6], int *s_ok,
+                                            u32x4 *xb, int tid, int lane, int sl, int kw, int ws,
+                                            int64_t wrow, const Outer<T> &oL, const Outer<T> &oR,
+                                            const T (&de)[NI], const T (&ga)[NI],
+                                            const T (&al)[NI]) {
+    constexpr int SW = IL ? 64 / S : 64;
+    constexpr int R = SW * NW * M;
+    const int K = p.K;
+    const int64_t n = p.n;
+    const int64_t brow = (int64_t)j * R;
+    const int64_t len = IL ? R : ((n - brow < R) ? n - brow : R);
+    const T *fg = p.forms + (int64_t)gi * K * S * 6;
+    if (tid == 0) {
+        int ok = 1;
+        for (int polls = 0; __hip_atomic_load(p.ctl + kCoopCtlStride * gi, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT) < K;) {
+            if (++polls > p.spin_limit) { ok = 0; break; }
+            __builtin_amdgcn_s_sleep(TRI_COOP_POLL_SLEEP);
+        }
+        *s_ok = ok;
+    }
+    __syncthreads();
+    const bool ok = *s_ok != 0;
+    for (int e = tid; e < K * S * 6; e += 64 * NW) {
+        const int k = e / (S * 6), r = e - k * (S * 6);
+        sh_k[r / 6][k][r % 6] =
+            ok ? __hip_atomic_load(fg + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : T(0);
+    }
+    __syncthreads();
+    if (tid < S) {
+        T U, V;
+        block_chain<T>(sh_k[tid], K, j, U, V);
+        chain_with_ends<T>(blk[tid], sh_uv[tid], NW, U, V);
+    }
+    __syncthreads();
+    const T u = sh_uv[sl][ws][0], v = sh_uv[sl][ws][1];
+    const T xL = oL.r - oL.p * u - oL.q * v;
+    const T xR = oR.r - oR.p * u - oR.q * v;
+    T xs[M];
+    xs[0] = xL;
+    xs[M - 1] = xR;
+    T xn = xR;
+#pragma unroll
+    for (int q = M - 3; q >= 0; --q) {
+        xn = (de[q] - al[q] * xL) - ga[q] * xn;
+        xs[q + 1] = xn;
+    }
+    if (!ok) {
+#pragma unroll
+        for (int q = 0; q < M; ++q) xs[q] = __builtin_nan("");
+    }
+    __amdgpu_buffer_rsrc_t rw[4];
+    coop_rsrc<T, IL>(p, gi, brow, rw);
+    if constexpr (IL)
+        il_block_store<T, M, SW, S, true>(rw[3], brow + wrow, p.rstride, (int64_t)gi * S, p.batch,
+                                          lane, sl, kw, xb, xs);
+    else
+        contig_block_store<T, M, true>(rw[3], wrow, len, lane, xb, xs);
+    __syncthreads();                           // LDS of this task free
+}
+
+// The same solve with the wait taken one task later: a workgroup reduces and
+// publishes task t + P before it waits for the group of task t, so the time the
+// group's slowest member needs is covered by a whole task's loads and arithmetic
+// instead of idling.  The lane state of the pending task (its outer forms and the
+// sweep coefficients of its rows, 3(M-2) + 6 values per lane) stays in registers.
+// Every wait for task t still comes after the workgroup has published task t + P,
+// and every member of a group publishes it before waiting on anything later, so
+// with all P workgroups resident every wait ends.
+template <typename T, int M, int NW, int S, bool IL, int KMAX>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)))
+    coop_pipe_kernel(CoopParams<T> p) {
+    using Sh = TileShape<T, M, NW, S, IL, !IL, false, false, !IL, true, true>;
+    constexpr int SW = IL ? 64 / S : 64;
+    constexpr int R = SW * NW * M;
+    constexpr int NQ = M * (int)sizeof(T) / 16;
+    constexpr int NI = Sh::NI;
+    static_assert(IL || S == 1, "contiguous groups hold one system");
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int sl = IL ? lane % S : 0;
+    const int kw = IL ? lane / S : lane;
+    const int ws = wave;
+    const int64_t wrow = (int64_t)ws * SW * M;
+    __shared__ u32x4 xbuf[NW][64 * M * (int)sizeof(T) / 16];
+    __shared__ T sh_blk[2][S][NW][6];
+    __shared__ T sh_uv[S][NW][2];
+    __shared__ T sh_k[S][KMAX][6];
+    __shared__ int s_ok;
+
+    const int K = p.K;
+    const int P = (int)gridDim.x;
+    const int ntask = p.ngroups * K;
+    const int64_t n = p.n;
+    int task = (int)blockIdx.x;
+    u32x4 g[4][NQ];
+    if (task < ntask) {
+        const int g0 = task / K, j0 = task - (task / K) * K;
+        __amdgpu_buffer_rsrc_t rr[4];
+        coop_rsrc<T, IL>(p, g0, (int64_t)j0 * R, rr);
+        if constexpr (IL) il_block_issue<T, M, SW, S, true>(rr, (int64_t)j0 * R + wrow, p.rstride,
+                                                            (int64_t)g0 * S, lane, g);
+        else contig_block_issue<T, M, true>(rr, wrow, lane, g);
+    }
+    // the pending task: published, not yet finished
+    int pend = -1, par = 0;
+    Outer<T> poL{}, poR{};
+    T pde[NI], pga[NI], pal[NI];
+    for (; task < ntask; task += P) {
+        const int gi = task / K, j = task - gi * K;
+        const int64_t brow = (int64_t)j * R;
+        const int64_t row0 = brow + wrow + (int64_t)kw * M;
+        const int64_t len = IL ? R : ((n - brow < R) ? n - brow : R);
+        T rows[4][M];
+        if constexpr (IL) {
+            il_block_transpose<T, M, S>(g, lane, sl, kw, xbuf[wave], rows);
+        } else {
+            contig_block_transpose<T, M>(g, lane, xbuf[wave], rows);
+#pragma unroll
+            for (int q = 0; q < M; ++q)
+                if (wrow + kw * M + q >= len) {
+                    rows[0][q] = 0; rows[1][q] = 1; rows[2][q] = 0; rows[3][q] = 0;
+                }
+        }
+        T al_r[NI];
+        SweepOut<T, NI> o;
+        RegLoader<T, Sh::CH, M> ld{rows};
+        lane_sweep<T, Sh>(o, ld, row0, n, [&](int q, T x) { al_r[q] = x; });
+        const T DLr = o.bL - o.cL * o.SA, ELr = o.cL * o.PI, FLr = o.dL - o.cL * o.SD;
+        const T ARr = -(o.aR * o.alp), DRr = o.bR - o.aR * o.g, FRr = o.dR - o.aR * o.del;
+        Outer<T> oL, oR;
+        segment_outer(o.aL, DLr, ELr, FLr, ARr, DRr, o.cR, FRr, oL, oR);
+        wm_wave_merges<T, SW, (IL ? S : 1)>(oL, oR, lane, sl, kw, SW, false, nullptr);
+        if (kw == 0) {
+            sh_blk[par][sl][ws][0] = oL.r; sh_blk[par][sl][ws][1] = oL.p; sh_blk[par][sl][ws][2] = oL.q;
+        }
+        if (kw == SW - 1) {
+            sh_blk[par][sl][ws][3] = oR.r; sh_blk[par][sl][ws][4] = oR.p; sh_blk[par][sl][ws][5] = oR.q;
+        }
+        __syncthreads();
+        T *fg = p.forms + (int64_t)gi * K * S * 6;
+        if (tid < S) {
+            Outer<T> F, E;
+            fold_blocks<T>(sh_blk[par][tid], NW, F, E);
+            T *f = fg + ((int64_t)j * S + tid) * 6;
+            const T vals[6] = {F.r, F.p, F.q, E.r, E.p, E.q};
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                __hip_atomic_store(f + q, vals[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+        __syncthreads();
+        if (tid == 0)
+            __hip_atomic_fetch_add(p.ctl + kCoopCtlStride * gi, 1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        const int next = task + P;
+        if (next < ntask) {
+            const int g1 = next / K, j1 = next - (next / K) * K;
+            __amdgpu_buffer_rsrc_t rr[4];
+            coop_rsrc<T, IL>(p, g1, (int64_t)j1 * R, rr);
+            if constexpr (IL) il_block_issue<T, M, SW, S, true>(rr, (int64_t)j1 * R + wrow,
+                                                                p.rstride, (int64_t)g1 * S, lane, g);
+            else contig_block_issue<T, M, true>(rr, wrow, lane, g);
+        }
+        if (pend >= 0) {
+            const int pg = pend / K;
+            coop_finish<T, M, NW, S, IL, KMAX, NI>(p, pg, pend - pg * K, sh_blk[par ^ 1], sh_uv,
+                                                   sh_k, &s_ok, xbuf[wave], tid, lane, sl, kw, ws,
+                                                   wrow, poL, poR, pde, pga, pal);
+        }
+        pend = task;
+        par ^= 1;
+        poL = oL;
+        poR = oR;
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            pde[q] = o.de[q];
+            pga[q] = o.ga[q];
+            pal[q] = al_r[q];
+        }
+    }
+    if (pend >= 0) {
+        const int pg = pend / K;
+        coop_finish<T, M, NW, S, IL, KMAX, NI>(p, pg, pend - pg * K, sh_blk[par ^ 1], sh_uv, sh_k,
+                                               &s_ok, xbuf[wave], tid, lane, sl, kw, ws, wrow, poL,
+                                               poR, pde, pga, pal);
+    }
+}
+
 // Clears the group arrival counters before a solve.
 __global__ void coop_reset_kernel(int32_t *ctl, int32_t count);
 
@@ -379,6 +573,7 @@ template <typename T> struct CoopEntry {
     bool IL;
     int R;       // rows per block
     int kmax;    // most blocks per system (LDS for the block chain)
+    bool pipe;   // coop_pipe_kernel by default
     hipError_t (*launch)(const CoopParams<T> &, int grid, hipStream_t);
     int (*max_grid)();   // co-resident workgroups on the current device (0: unknown)
 };
