@@ -134,6 +134,26 @@ int tridiag_algo_layout(int precision, int algo);
 int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch,
                   void *a, void *b, void *c, void *d, void *stream);
 
+/* The reference's own streaming form of algorithm ids 3-10, one wave per
+ * system, with the reference's memory semantics (c' into c and d' into d on the
+ * way down, x into d on the way up):
+ *   3 / 5  PCR-pThomas (tridiagonal_wm_pcr.cu:886-1018, 1160-1282): 6 PCR steps
+ *          over a sliding window of 64-row blocks, Thomas along the 64 stride-64
+ *          subsystems;
+ *   4 / 6  PCR-half-pThomas (:1020-1158, 1284-1431): 5 steps, 32 stride-32
+ *          subsystems over two chained half-waves;
+ *   7 / 9  WM-pGE (:17-205, 424-646): 128-row chunks, 6 Wang-Mou merge rounds to
+ *          64-row blocks, partial Gaussian elimination block after block;
+ *   8 / 10 WM-half-pGE (:207-422, 648-884): 5 rounds, 32-row blocks;
+ * the shared-memory ids (3, 4, 7, 8) exchange rows between lanes through LDS,
+ * the register ids (5, 6, 9, 10) through ds_bpermute.  tridiag_solve runs the
+ * on-chip partition kernel for these ids unless $TRIDIAG_FORM=bulk; this entry
+ * always runs the bulk-row form.  Layouts 0 and 1; any n_eqt >= 1 (the
+ * reference needs n % 64 for WM and n % 32 for PCR).  Other ids and the
+ * interleaved layout return TRIDIAG_EUNSUPPORTED.  Asynchronous on `stream`. */
+int tridiag_solve_bulk(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch,
+                       void *a, void *b, void *c, void *d, void *stream);
+
 /* tridiag_solve without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
@@ -231,6 +251,20 @@ int tridiag_convert_layout(int precision, int src_layout, int dst_layout, int64_
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *tridiag_last_error(void);
+
+/* Division policy this library was built with; replaces the reference's
+ * compile-time division macros (Tridiagonal_CUDA/intrin.h:23-87, selected by
+ * -D flags in its Makefile).  Every variant exports the same symbols, so a caller
+ * picks a policy by linking (or loading) the matching library:
+ *   TRIDIAG_DIV_REFINED  libtridiag_amd.so          hardware reciprocal + Newton
+ *                                                    steps (FP32 one, FP64 two)
+ *   TRIDIAG_DIV_IEEE     libtridiag_amd_div_ieee.so correctly rounded reciprocal
+ *   TRIDIAG_DIV_FAST     libtridiag_amd_div_fast.so hardware reciprocal alone
+ *                                                    (FP32), one Newton step (FP64) */
+#define TRIDIAG_DIV_REFINED 0
+#define TRIDIAG_DIV_IEEE 1
+#define TRIDIAG_DIV_FAST 2
+int tridiag_div_mode(void);
 
 /* Kernel configuration the dispatcher would use for arrays with 16-byte
  * aligned rows (for reports and tuning).

@@ -6,7 +6,7 @@ tensors, exactly like the reference's NAME_{single,double}(a, b, c, d, n, batch)
 calls, plus the interleaved layout, ragged batches and the adjoint.
 """
 from ._capi import (ALGO_AUTO, CONTIGUOUS, DOUBLE, INTERLEAVED, PACKED, SINGLE,  # noqa: F401
-                    TridiagError, algo_names, check, lib, plan, ragged_bucket,
+                    TridiagError, algo_names, check, div_mode, lib, plan, ragged_bucket,
                     ragged_shapes, tiles)
-from .solver import (RaggedPlan, convert_layout, solve, solve_adjoint,  # noqa: F401
+from .solver import (RaggedPlan, convert_layout, solve, solve_adjoint, solve_bulk,  # noqa: F401
                      solve_fwd_bwd, solve_ragged, TridiagonalSolve, tridiagonal_solve)

@@ -7,7 +7,13 @@ import ctypes
 import os
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG, "lib", "libtridiag_amd.so")
+
+# Division policies (include/tridiagonal.h, TRIDIAG_DIV_*): one library file each,
+# with the same symbols.  $TRIDIAG_DIV_MODE picks the file this binding loads.
+DIV_MODES = {"refined": 0, "ieee": 1, "fast": 2}
+_DIV_MODE = os.environ.get("TRIDIAG_DIV_MODE", "refined")
+LIB_PATH = os.path.join(_PKG, "lib", "libtridiag_amd.so" if _DIV_MODE == "refined"
+                        else "libtridiag_amd_div_%s.so" % _DIV_MODE)
 
 SINGLE, DOUBLE = 0, 1
 CONTIGUOUS, PACKED, INTERLEAVED = 0, 1, 2
@@ -33,7 +39,7 @@ EXTENDED_SYMBOLS = [
     "tridiag_tile_info", "tridiag_solve_tile", "tridiag_solve_tile_iface", "tridiag_plan_kind",
     "tridiag_ragged_shape_count",
     "tridiag_ragged_shape_info", "tridiag_ragged_bucket", "tridiag_ragged_plan",
-    "tridiag_ragged_solve", "tridiag_convert_layout",
+    "tridiag_ragged_solve", "tridiag_convert_layout", "tridiag_div_mode", "tridiag_solve_bulk",
 ]
 
 
@@ -51,10 +57,16 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
+    if _DIV_MODE not in DIV_MODES:
+        raise ValueError("TRIDIAG_DIV_MODE must be one of %s, not %r" % (sorted(DIV_MODES), _DIV_MODE))
     if not os.path.exists(LIB_PATH):
-        raise ImportError("libtridiag_amd.so not built (run python -m tridiagonalsolver_amd.build)")
+        raise ImportError("%s not built (run python -m tridiagonalsolver_amd.build%s)" %
+                          (os.path.basename(LIB_PATH),
+                           "" if _DIV_MODE == "refined" else " --div-mode " + _DIV_MODE))
     L = ctypes.CDLL(LIB_PATH)
     vp, i64, ci = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+    L.tridiag_div_mode.argtypes = []
+    L.tridiag_div_mode.restype = ci
     L.tridiag_algo_count.argtypes = [ci]
     L.tridiag_algo_count.restype = ci
     L.tridiag_algo_name.argtypes = [ci, ci]
@@ -63,6 +75,8 @@ def lib():
     L.tridiag_algo_layout.restype = ci
     L.tridiag_solve.argtypes = [ci, ci, ci, i64, i64, vp, vp, vp, vp, vp]
     L.tridiag_solve.restype = ci
+    L.tridiag_solve_bulk.argtypes = [ci, ci, ci, i64, i64, vp, vp, vp, vp, vp]
+    L.tridiag_solve_bulk.restype = ci
     L.tridiag_solve_noalloc.argtypes = [ci, ci, ci, i64, i64, vp, vp, vp, vp, vp]
     L.tridiag_solve_noalloc.restype = ci
     L.tridiag_ragged_work_size.argtypes = [i64]
@@ -118,7 +132,7 @@ def algo_names(precision):
             for i in range(L.tridiag_algo_count(precision))]
 
 
-PLAN_KINDS = ["lane", "thomas", "tile", "cooperative", "chunked", "via_contiguous"]
+PLAN_KINDS = ["lane", "thomas", "tile", "cooperative", "chunked", "via_contiguous", "bulk_row"]
 
 
 def plan(precision, algo, layout, n, batch):
@@ -179,3 +193,9 @@ def ragged_bucket(precision, n):
     sh, fi = ctypes.c_int(), ctypes.c_int()
     check(lib().tridiag_ragged_bucket(precision, n, ctypes.byref(sh), ctypes.byref(fi)))
     return sh.value, fi.value
+
+
+def div_mode():
+    """Name of the division policy of the loaded library ("refined", "ieee", "fast")."""
+    code = lib().tridiag_div_mode()
+    return {v: k for k, v in DIV_MODES.items()}[code]

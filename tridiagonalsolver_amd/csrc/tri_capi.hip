@@ -401,6 +401,13 @@ static int32_t coop_spin_limit() {
     return v;
 }
 
+// Which cooperative kernel runs: the entry's choice, or $TRIDIAG_COOP_PIPE (1: the
+// one that waits one task later, 0: the one that waits before its next task).
+static int32_t coop_pipelined(bool entry_default) {
+    static const int64_t v = env_int("TRIDIAG_COOP_PIPE", -1);
+    return v < 0 ? (entry_default ? 1 : 0) : (v != 0 ? 1 : 0);
+}
+
 // The entry of the cooperative table for (layout, n): interleaved blocks must
 // divide n; K = blocks per system in [2, kmax].  Contiguous systems take it up to
 // 16 blocks: measured on MI355X at 2^24 unknowns against the chunked solve, FP32
@@ -422,6 +429,47 @@ static const CoopEntry<T> *choose_coop(bool il, int64_t n, int &K) {
     return nullptr;
 }
 
+// Bulk-row form of a reference id (tri_bulk.hip): form 0 PCR-pThomas, 1
+// PCR-half-pThomas, 2 WM-pGE, 3 WM-half-pGE, and whether its exchanges go through
+// LDS (the shared-memory ids) or ds_bpermute (the register ids).  False for ids
+// without one.
+static bool bulk_form(int algo, int &form, bool &lds) {
+    switch (algo) {
+    case kPCR5Shmem: form = 0; lds = true; return true;
+    case kPCR4Shmem: form = 1; lds = true; return true;
+    case kPCR5Reg: form = 0; lds = false; return true;
+    case kPCR4Reg: form = 1; lds = false; return true;
+    case kWM5Shmem: form = 2; lds = true; return true;
+    case kWM4Shmem: form = 3; lds = true; return true;
+    case kWM5Reg: form = 2; lds = false; return true;
+    case kWM4Reg: form = 3; lds = false; return true;
+    default: return false;
+    }
+}
+// $TRIDIAG_FORM=bulk: ids 3-10 in the contiguous and packed layouts run their
+// bulk-row forms (also through the reference symbols) instead of the partition
+// kernel.
+static bool bulk_forms_requested() {
+    static const bool v = [] {
+        const char *e = getenv("TRIDIAG_FORM");
+        return e && std::string(e) == "bulk";
+    }();
+    return v;
+}
+template <typename T>
+static hipError_t bulk_launch(int algo, const T *a, const T *b, T *c, T *d, int64_t n,
+                              int64_t batch, int64_t sstride, int32_t bb, hipStream_t st) {
+    int form = 0;
+    bool lds = false;
+    if (!bulk_form(algo, form, lds)) return hipErrorInvalidValue;
+    if constexpr (sizeof(T) == 4)
+        return launch_bulk_f32(form, lds, (const float *)a, (const float *)b, (float *)c,
+                               (float *)d, n, batch, sstride, bb, st);
+    else
+        return launch_bulk_f64(form, lds, (const double *)a, (const double *)b, (double *)c,
+                               (double *)d, n, batch, sstride, bb, st);
+}
+
 // How solve_impl solves one batch (decided without launching anything).
 enum PlanKind {
     kPlanLane = 0,           // one system per lane, in registers (short systems)
@@ -430,6 +478,7 @@ enum PlanKind {
     kPlanCoop = 3,           // on chip, K workgroups per group of systems
     kPlanChunked = 4,        // three passes over chunks through global memory
     kPlanViaContiguous = 5,  // interleaved systems solved through a contiguous copy
+    kPlanBulk = 6,           // the reference's bulk-row form of ids 3-10, one wave per system
 };
 template <typename T> struct SolvePlan {
     int kind = kPlanTile;
@@ -492,7 +541,8 @@ static int coop_run(const SolvePlan<T> &pl, int layout, int64_t n, int64_t batch
     }
     CoopParams<T> p{a, b, c, d, n, batch, il ? 1 : sstride, il ? batch : 1,
                     (int32_t)(il ? n * batch * (int64_t)sizeof(T) : 0), (int32_t)pl.K,
-                    (int32_t)ngroups, coop_spin_limit(), reinterpret_cast<T *>(scratch),
+                    (int32_t)ngroups, coop_spin_limit(), coop_pipelined(e.pipe),
+                    reinterpret_cast<T *>(scratch),
                     reinterpret_cast<int32_t *>(reinterpret_cast<char *>(scratch) +
                                                 pl.forms_bytes)};
     int rc = TRIDIAG_OK;
@@ -570,6 +620,12 @@ static SolvePlan<T> make_plan(int algo, int layout, int64_t n, int64_t batch, in
     }
     if (algo == kThomas) {
         pl.kind = kPlanThomas;
+        return pl;
+    }
+    int form = 0;
+    bool lds = false;
+    if (!il && bb != 0 && bulk_forms_requested() && bulk_form(algo, form, lds)) {
+        pl.kind = kPlanBulk;
         return pl;
     }
     // rows beyond one workgroup: K workgroups per group of systems, on chip
@@ -656,6 +712,10 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
                                 batch, sstride, rstride, st);
         return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "Thomas launch");
     }
+    case kPlanBulk: {
+        const hipError_t e = bulk_launch<T>(algo, a, b, c, d, n, batch, sstride, bb, st);
+        return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "bulk-row launch");
+    }
     case kPlanCoop:
         return coop_run<T>(pl, layout, n, batch, sstride, a, b, c, d, st, fx);
     case kPlanChunked:
@@ -677,6 +737,43 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
                                               " needs more LDS than the kernel for n_eqt=" +
                                               std::to_string(n) + " leaves");
     return err == hipSuccess ? TRIDIAG_OK : hip_fail(err, "tile launch");
+}
+
+// The bulk-row form of algorithm `algo` (ids 3-10) in the contiguous or packed
+// layout; batches whose arrays reach 2 GiB run in sub-2-GiB launches.
+template <typename T>
+static int bulk_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b, T *c, T *d,
+                     hipStream_t st) {
+    int form = 0;
+    bool lds = false;
+    if (!bulk_form(algo, form, lds))
+        return fail(TRIDIAG_EUNSUPPORTED, "algorithm " + std::to_string(algo) +
+                                              " has no bulk-row form (ids 3-10 do)");
+    if (n < 1) return fail(TRIDIAG_EINVAL, "n_eqt must be >= 1");
+    if (batch < 0) return fail(TRIDIAG_EINVAL, "n_batch must be >= 0");
+    if (layout == kInterleaved)
+        return fail(TRIDIAG_EUNSUPPORTED, "the bulk-row forms stream one system per wave: "
+                                          "layouts 0 and 1 only");
+    if (layout != kContiguous && layout != kPacked)
+        return fail(TRIDIAG_EINVAL, "layout must be 0, 1 or 2");
+    if (!a || (layout == kContiguous && (!b || !c || !d)))
+        return fail(TRIDIAG_EINVAL, "null coefficient pointer");
+    if (batch == 0) return TRIDIAG_OK;
+    const int64_t sstride = layout == kPacked ? 4 * n : n;
+    const int64_t per = std::max<int64_t>(1, ((int64_t)0x7fffff00 / (int64_t)sizeof(T)) / sstride);
+    if (sstride * (int64_t)sizeof(T) > (int64_t)0x7fffff00)
+        return fail(TRIDIAG_EUNSUPPORTED, "one system of 2 GiB or more");
+    for (int64_t s0 = 0; s0 < batch; s0 += per) {
+        const int64_t cnt = std::min(per, batch - s0);
+        T *a0 = a + s0 * sstride;
+        T *b0 = layout == kPacked ? a0 + n : b + s0 * sstride;
+        T *c0 = layout == kPacked ? a0 + 2 * n : c + s0 * sstride;
+        T *d0 = layout == kPacked ? a0 + 3 * n : d + s0 * sstride;
+        const int32_t bb = buffer_extent<T>(n, cnt, sstride, layout);
+        const hipError_t e = bulk_launch<T>(algo, a0, b0, c0, d0, n, cnt, sstride, bb, st);
+        if (e != hipSuccess) return hip_fail(e, "bulk-row launch");
+    }
+    return TRIDIAG_OK;
 }
 
 // ---------------------------------------------------------------- no-allocation solve
@@ -1102,6 +1199,21 @@ int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_
                               (double *)c, (double *)d, st);
 }
 
+int tridiag_solve_bulk(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch,
+                       void *a, void *b, void *c, void *d, void *stream) {
+    g_last_error.clear();
+    if (precision != TRIDIAG_SINGLE && precision != TRIDIAG_DOUBLE)
+        return fail(TRIDIAG_EINVAL, "precision must be 0 (single) or 1 (double)");
+    if (algo < 0 || algo >= algo_count(precision))
+        return fail(TRIDIAG_EINVAL, "unknown algorithm id " + std::to_string(algo));
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == TRIDIAG_SINGLE)
+        return bulk_impl<float>(algo, layout, n_eqt, n_batch, (float *)a, (float *)b, (float *)c,
+                                (float *)d, st);
+    return bulk_impl<double>(algo, layout, n_eqt, n_batch, (double *)a, (double *)b, (double *)c,
+                             (double *)d, st);
+}
+
 int tridiag_solve_noalloc(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch,
                           void *a, void *b, void *c, void *d, void *stream) {
     g_last_error.clear();
@@ -1334,6 +1446,8 @@ int tridiag_convert_layout(int precision, int src_layout, int dst_layout, int64_
 }
 
 const char *tridiag_last_error(void) { return g_last_error.c_str(); }
+
+int tridiag_div_mode(void) { return tri::kDivMode; }
 
 // ======================================================================= Part 1
 #define REF_SYMBOL(NAME, T, ALGO, LAYOUT)                                                  \

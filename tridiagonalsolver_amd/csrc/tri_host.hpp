@@ -90,6 +90,15 @@ hipError_t launch_thomas_rows_f32(const float *a, const float *b, float *c, floa
 hipError_t launch_thomas_rows_f64(const double *a, const double *b, double *c, double *d,
                                   int64_t n, int64_t batch, int64_t sstride, int32_t bb,
                                   hipStream_t st);
+// Bulk-row forms of the reference's streaming solvers (tri_bulk.hip), one wave per
+// system; form 0 PCR-pThomas, 1 PCR-half-pThomas, 2 WM-pGE, 3 WM-half-pGE; lds:
+// lane exchanges through LDS instead of ds_bpermute.  Arrays below 2 GiB (bb =
+// byte extent), sstride >= n.
+hipError_t launch_bulk_f32(int form, bool lds, const float *a, const float *b, float *c, float *d,
+                           int64_t n, int64_t batch, int64_t sstride, int32_t bb, hipStream_t st);
+hipError_t launch_bulk_f64(int form, bool lds, const double *a, const double *b, double *c,
+                           double *d, int64_t n, int64_t batch, int64_t sstride, int32_t bb,
+                           hipStream_t st);
 // Composed adjoint (systems the fused adjoint kernels do not hold): the shifted
 // coefficients of A^T into ap / cp, g into lam, and d into xo when d is given;
 // then, after the solves, the coefficient gradients from x and lambda.

@@ -61,6 +61,19 @@ def solve(a, b, c, d, algo=ALGO_AUTO, layout=CONTIGUOUS, n=None, batch=None):
     return a if layout == PACKED else d
 
 
+def solve_bulk(a, b, c, d, algo, layout=CONTIGUOUS, n=None, batch=None):
+    """The reference's bulk-row form of algorithm `algo` (ids 3-10: PCR-pThomas,
+    PCR-half-pThomas, WM-pGE, WM-half-pGE, shared-memory or register exchanges),
+    in place like solve(): x into d, c' into c.  Layouts 0 and 1."""
+    ts = [a] if layout == PACKED else [a, b, c, d]
+    _check_tensors(ts)
+    n, batch = _shape(a if layout == PACKED else d, layout, n, batch)
+    prec = _precision(a)
+    args = [_ptr(a)] + ([_ptr(a)] * 3 if layout == PACKED else [_ptr(b), _ptr(c), _ptr(d)])
+    check(lib().tridiag_solve_bulk(prec, algo, layout, n, batch, *args, _stream_handle(a.device)))
+    return a if layout == PACKED else d
+
+
 def convert_layout(arrays, src_layout, dst_layout, n=None, batch=None):
     """Copies coefficient arrays into another layout (a permutation, bit-exact).
 
