@@ -59,30 +59,32 @@ def _settle(torch, fn, min_ms=SETTLE_MS):
             return
 
 
-def time_solver(torch, T, sets, n, batch, steps, warmup, algo, layout):
-    """Times `steps` back-to-back solves through the C-ABI on torch's current stream.
-    Returns (wall ms per step, ms per launch from HIP events around the timed region)."""
+def time_solver(torch, T, sets, n, batch, steps, warmup, algo, layout, bulk=False):
+    """Times `steps` back-to-back solves through the C-ABI on torch's current stream
+    (tridiag_solve, or tridiag_solve_bulk with bulk=True).  Returns (wall ms per
+    step, ms per launch from HIP events around the timed region)."""
     import ctypes
     stream = torch.cuda.current_stream()
     prec = 0 if sets[0][0].dtype == torch.float32 else 1
     L = T.lib()
+    entry = L.tridiag_solve_bulk if bulk else L.tridiag_solve
     h = ctypes.c_void_p(stream.cuda_stream)
     args = [(prec, algo, layout, n, batch) + tuple(ctypes.c_void_p(t.data_ptr()) for t in s) + (h,)
             for s in sets]
     for k in range(warmup):
-        T.check(L.tridiag_solve(*args[k % len(args)]))
+        T.check(entry(*args[k % len(args)]))
     turn = [0]
 
     def rotate():                              # settle on the rotated sets, as timed
         turn[0] += 1
-        return L.tridiag_solve(*args[turn[0] % len(args)])
+        return entry(*args[turn[0] % len(args)])
     _settle(torch, rotate)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     rcs = 0
     t0 = time.perf_counter()
     e0.record(stream)
     for k in range(steps):
-        rcs |= L.tridiag_solve(*args[k % len(args)])
+        rcs |= entry(*args[k % len(args)])
     e1.record(stream)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3 / steps
@@ -91,14 +93,17 @@ def time_solver(torch, T, sets, n, batch, steps, warmup, algo, layout):
     return wall, kern
 
 
-def check_parity(torch, T, dtype, n, batch, algo, layout, tol):
+def check_parity(torch, T, dtype, n, batch, algo, layout, tol, bulk=False):
     """Solves one fresh set and compares every system with the FP64 oracle."""
     import numpy as np
     import oracle
     a, b, c, d = oracle.gen_dd(n, batch, layout=layout, seed=99)
     arrs = [v.astype(dtype) for v in (a, b, c, d)]
     ts = [torch.from_numpy(v).cuda() for v in arrs]
-    T.solve(*ts, algo=algo, layout=layout, n=n, batch=batch)
+    if bulk:
+        T.solve_bulk(*ts, algo, layout=layout, n=n, batch=batch)
+    else:
+        T.solve(*ts, algo=algo, layout=layout, n=n, batch=batch)
     x = ts[3].cpu().numpy().astype(np.float64)
     xref = oracle.thomas(*(v.astype(np.float64) for v in arrs), n, batch, layout=layout)
     st = oracle.errstats(x, xref, n, batch, layout)
@@ -327,6 +332,42 @@ def run_pcr(torch, T, steps, warmup, algo=5):
                          plan=T.plan(0 if w == 4 else 1, algo, layout, n, batch), parity=par)
         log("pcr", name, json.dumps(out[name]))
     return {"algo": algo, "name": "reg PCR-pThomas", **out}
+
+
+BULK_IDS = [(3, "shmem PCR-pThomas"), (5, "reg PCR-pThomas"), (7, "shmem WM-pGE"),
+            (9, "reg WM-pGE")]
+
+
+def run_bulk(torch, T, steps, warmup):
+    """The reference's own streaming forms of PCR-pThomas and WM-pGE
+    (tridiag_solve_bulk: one wave per system, c' and d' through memory, 9 words
+    per unknown) at n=4096 x 4096 contiguous, FP32 and FP64, beside the partition
+    kernel tridiag_solve runs for the same ids."""
+    import numpy as np
+    n, batch = N_EQT, N_BATCH
+    out = {}
+    for name, dtname, layout in WORKLOADS[:2]:
+        dtype = np.dtype(dtname).type
+        w = np.dtype(dtype).itemsize
+        log("prepare bulk", name)
+        sets = make_sets(torch, dtype, n, batch, N_SETS, seed0=11, layout=layout)
+        res = {}
+        for algo, label in BULK_IDS:
+            wall, kern = time_solver(torch, T, sets, n, batch, steps, warmup, algo, layout,
+                                     bulk=True)
+            res[str(algo)] = dict(name=label, kernel_ms=kern, ms_per_step=wall,
+                                  unknowns_per_s=n * batch / (wall * 1e-3),
+                                  hbm_frac=round(5 * w * n * batch / (kern * 1e-3) / 1e9 /
+                                                 HBM_PEAK_GBS, 4),
+                                  traffic_rate_gbs=round(9 * w * n * batch / (kern * 1e-3) / 1e9, 1))
+        del sets
+        torch.cuda.empty_cache()
+        for algo, _ in BULK_IDS:
+            res[str(algo)]["parity"] = check_parity(torch, T, dtype, n, batch, algo, layout,
+                                                    1e-5 if w == 4 else 1e-12, bulk=True)
+        out[name] = res
+        log("bulk", name, json.dumps(res))
+    return out
 
 
 def _event_time(torch, fn, steps, stream):
@@ -727,6 +768,12 @@ def run_n1024(torch, T, steps, warmup, with_cpu, budget_s=2.0):
     sets = make_sets(torch, np.float64, n, batch, N_SETS, 21, 0)
     wall, kern = time_solver(torch, T, sets, n, batch, steps, warmup, -1, 0)
     par = check_parity(torch, T, np.float64, n, batch, -1, 0, 1e-12)
+    # the Thomas id itself (one wave per system, 9 words per unknown), event-timed
+    twall, tkern = time_solver(torch, T, sets, n, batch, steps, warmup, 1, 0)
+    thomas = {"algo": 1, "plan": T.plan(1, 1, 0, n, batch), "ms_per_step": twall,
+              "kernel_ms": tkern, "unknowns_per_s": U / (tkern * 1e-3),
+              "traffic_rate_gbs_9w": round(9 * 8 * U / (tkern * 1e-3) / 1e9, 1),
+              "parity": check_parity(torch, T, np.float64, n, batch, 1, 0, 1e-12)}
     L = T.lib()
     src = sets[0]
     work = [t.clone() for t in src]
@@ -751,7 +798,7 @@ def run_n1024(torch, T, steps, warmup, with_cpu, budget_s=2.0):
          "plan": T.plan(1, -1, 0, n, batch), "ms_per_step": wall, "kernel_ms": kern,
          "unknowns_per_s": U / (kern * 1e-3),
          "hbm_frac_mall_resident": 5 * 8 * U / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS,
-         "parity": par, "reference_style_sync_call": ref_style}
+         "parity": par, "thomas": thomas, "reference_style_sync_call": ref_style}
     del sets, work
     torch.cuda.empty_cache()
     if with_cpu:
@@ -845,6 +892,7 @@ def main():
         res[name] = run_workload(torch, T, name, dtname, layout, args.steps, args.warmup)
     main_r = res["f32_contiguous"]
     pcr = run_pcr(torch, T, args.steps, args.warmup)
+    bulk = run_bulk(torch, T, args.steps, args.warmup)
     large = run_large(torch, T, args.steps, args.warmup)
     ragged = run_ragged(torch, T, args.steps, args.warmup)
     fwdbwd = run_fwd_bwd(torch, T, args.steps, args.warmup)
@@ -878,6 +926,7 @@ def main():
                                                "parity", "roofline")}
                       for k, v in res.items()},
         "pcr_4096x4096": pcr,
+        "bulk_row_forms_4096x4096": bulk,
         "large": large,
         "ragged": ragged,
         "n1024_f64": n1024,
