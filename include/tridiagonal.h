@@ -191,7 +191,9 @@ int tridiag_solve_noalloc(int precision, int algo, int layout, int64_t n_eqt, in
  *                         iface = -1 take each bucket's choice from the
  *                         autotuned table (tuning_gfx950.txt, lines
  *                         "R precision n_max shape iface"); a shape or iface
- *                         >= 0 forces it for every system.
+ *                         >= 0 forces it for every system (iface: an interface
+ *                         solver, or TRIDIAG_IFACE_PCR_DEPTH(d) for PCR +
+ *                         Thomas of depth d <= log2(64 * waves per system)).
  *   tridiag_ragged_solve  launches the solve for a planned work buffer (one
  *                         kernel, no host synchronisation, no allocation:
  *                         capturable into a graph and replayable).  Concurrent
@@ -298,7 +300,13 @@ int tridiag_solve_tile(int precision, int index, int layout, int64_t n_eqt, int6
                        void *a, void *b, void *c, void *d, void *stream);
 /* The same with the interface solver chosen: 0 Wang-Mou merges (what
  * tridiag_solve_tile uses), 1 Wang-Mou half, 2 PCR, 3 PCR + Thomas, 4 CR.  The
- * autotuner times each algorithm family on its own interface solver with it. */
+ * autotuner times each algorithm family on its own interface solver with it.
+ * For PCR + Thomas, TRIDIAG_IFACE_PCR_DEPTH(d) sets the PCR depth: d PCR steps on
+ * the interface rows, then Thomas along the 2^d subsystems left (1 <= d <=
+ * log2(P) for a tile of P = capacity / M lane segments per system; 0 or no depth
+ * keeps the default, PCR up to the wave's segment stride).  Other codes:
+ * TRIDIAG_EINVAL.  The same codes serve tridiag_ragged_plan's iface. */
+#define TRIDIAG_IFACE_PCR_DEPTH(d) (3 | ((d) << 4))
 int tridiag_solve_tile_iface(int precision, int index, int iface, int layout, int64_t n_eqt,
                              int64_t n_batch, void *a, void *b, void *c, void *d, void *stream);
 

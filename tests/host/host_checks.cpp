@@ -48,7 +48,7 @@ static void check_schedule(const std::vector<int64_t> &off, int prec, int fs, in
         const int shape = r[0], iface = r[1], count = r[2];
         CHECK(shape >= 0 && shape < kRaggedShapes, "shape %d", shape);
         if (shape < 0 || shape >= kRaggedShapes) return;
-        CHECK(iface >= 0 && iface < kRaggedIfaces, "iface %d", iface);
+        CHECK(iface_code_valid(iface, ragged_segments(shape)), "iface %d", iface);
         CHECK(count >= 1 && count <= kRaggedShapeTab[shape].S, "count %d", count);
         CHECK(ragged_cap(shape) <= last_cap, "buckets out of capacity order");
         last_cap = ragged_cap(shape);
@@ -85,7 +85,7 @@ static void ragged_cases() {
     auto dflt = [](int64_t, int &shape, int &iface) { shape = -1; iface = 0; };
     auto odd = [](int64_t n, int &shape, int &iface) {   // a tuning table choosing other shapes
         shape = (int)(n % kRaggedShapes);
-        iface = (int)(n % kRaggedIfaces);
+        iface = (int)(n % kRaggedIfaces) | (int)((n / 7) % 11) << 4;   // some depths invalid
     };
     for (int trial = 0; trial < 300; ++trial) {
         const int64_t n_sys = (int64_t)(rng() % 3000);
@@ -126,6 +126,12 @@ static void ragged_cases() {
     CHECK(ragged_schedule(2, 1, small.data(), -1, -1, dflt2, img, err) == 1, "precision");
     CHECK(ragged_schedule(0, 1, small.data(), kRaggedShapes, -1, dflt2, img, err) == 1, "shape id");
     CHECK(ragged_schedule(0, 1, small.data(), -1, kRaggedIfaces, dflt2, img, err) == 1, "iface id");
+    // PCR depths: up to log2 of the shape's segments (the shape of 8 waves: 512, depth 9)
+    CHECK(ragged_schedule(0, 1, small.data(), -1, 3 | 2 << 4, dflt2, img, err) == 0, "PCR depth 2");
+    CHECK(ragged_schedule(0, 1, small.data(), -1, 3 | 10 << 4, dflt2, img, err) == 1, "PCR depth 10");
+    CHECK(ragged_schedule(0, 1, small.data(), -1, 2 | 1 << 4, dflt2, img, err) == 1,
+          "depth on a family without one");
+    CHECK(!iface_code_valid(3 | 7 << 4, 64) && iface_code_valid(3 | 6 << 4, 64), "depth bound");
     CHECK(ragged_schedule(0, 1, nullptr, -1, -1, dflt2, img, err) == 1, "null offsets");
     img.clear();
     CHECK(ragged_schedule(0, 0, nullptr, -1, -1, dflt2, img, err) == 0 &&

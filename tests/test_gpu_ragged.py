@@ -139,6 +139,26 @@ def test_forced_shape_and_interface(prec, iface):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
+def test_pcr_depths(prec):
+    """PCR + Thomas at PCR depths 1 .. log2(64 NWS) on every shape (interface-solver
+    code 3 | depth << 4), and a depth past the shape's segments refused."""
+    import torch
+    import tridiagonalsolver_amd as T
+    dtype = DT[prec]
+    for sh in _shapes():
+        cap, top = sh["cap"], (64 * sh["NWS"]).bit_length() - 1
+        rng = np.random.default_rng(7 * sh["index"])
+        sizes = np.concatenate([[cap, 1, cap - 1], rng.integers(1, cap + 1, size=2 * sh["S"])])
+        off, arrs = _case(sizes, dtype, seed=sh["index"] + 3)
+        for depth in sorted({1, 2, top // 2, top - 1, top} - {0}):
+            announce("ragged pcr depth", prec, "shape", sh, "depth", depth)
+            _check(off, arrs, _run(off, arrs, shape=sh["index"], iface=3 | depth << 4), dtype)
+        with pytest.raises(T.TridiagError):
+            T.RaggedPlan(off, dtype=torch.float32 if prec == 0 else torch.float64,
+                         shape=sh["index"], iface=3 | (top + 1) << 4)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
 def test_bit_identical_to_plain_tile(prec):
     """A system solved in the persistent kernel equals, bit for bit, the same
     system solved by the plain tile launch of the same lane shape (M rows per

@@ -93,3 +93,37 @@ def test_every_tile_every_interface_solver(prec, interleaved):
                 fails.append((t, iface, n, st))
     assert ran > 0
     assert not fails, fails[:5]
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_pcr_depths(prec):
+    """PCR + Thomas with every PCR depth d = 1 .. log2(P) (interface-solver code
+    3 | d << 4: d PCR steps, then Thomas along the 2^d remaining subsystems) on a
+    contiguous and an interleaved tile of each precision; depths past log2(P) are
+    refused."""
+    import tridiagonalsolver_amd as T
+    picked = {}
+    for t in T.tiles(prec):
+        if t["wide_addr"] or t["padded"] or (prec == 0 and t["vec"] and t["cap"] % 4):
+            continue
+        key = (t["interleaved"], t["cap"] // t["M"])
+        if key[1] >= 64 and key not in picked:
+            picked[key] = t
+    assert picked
+    fails = []
+    for (il, P), t in sorted(picked.items())[:6]:
+        top = P.bit_length() - 1
+        for depth in range(1, top + 1):
+            announce("tile pcr depth", prec, t, "P", P, "depth", depth)
+            st, res = _run_tile(prec, t, t["cap"], 2 * t["S"] + 1, seed=depth, iface=3 | depth << 4)
+            if st is None:
+                continue
+            if not (st["nonfinite"] == 0 and st["relerr"] <= TOL[DT[prec]]):
+                fails.append((t, depth, st))
+        layout = 2 if il else 0
+        nul = ctypes.c_void_p(0)
+        assert T.lib().tridiag_solve_tile_iface(prec, t["index"], 3 | (top + 1) << 4, layout,
+                                                t["cap"], 1, nul, nul, nul, nul, nul) == _capi.EINVAL
+        assert T.lib().tridiag_solve_tile_iface(prec, t["index"], 2 | 1 << 4, layout, t["cap"], 1,
+                                                nul, nul, nul, nul, nul) == _capi.EINVAL
+    assert not fails, fails[:5]

@@ -23,7 +23,12 @@ the rows of its own interface solver first, then at family 0.  Tuning at sizes b
 Kernels are timed as back-to-back launches on rotated input sets, the way a
 caller streaming solves sees them.  --ragged writes lines "R precision n_max shape iface": the shape
 of the ragged persistent kernel and the interface solver for systems of up to
-n_max rows (above the previous line's n_max).
+n_max rows (above the previous line's n_max).  iface is an interface-solver code:
+the family in bits 0-3 and, for PCR + Thomas (family 3), the PCR depth in bits 4-7
+(the PCR steps before Thomas runs along the remaining subsystems; 0 = the
+default).  The ragged sweep times the PCR + Thomas family at every depth the
+shape allows (--pcr-depths limits the list), so a bucket's line also carries its
+PCR depth.
 """
 import argparse
 import ctypes
@@ -150,12 +155,24 @@ def _ragged_inputs(torch, prec, sizes, seed):
     return off, [a, b, c, d]
 
 
-def ragged_sweep(prec, kmin=5, kmax=13, rows_per_class=1 << 23, reps=10, seed=3):
+def iface_codes(nws, depths=None):
+    """Interface-solver codes a ragged shape of `nws` waves per system takes: the five
+    families at their defaults, then PCR + Thomas at PCR depths 1 .. log2(64 nws)
+    (or the given depths, where valid)."""
+    segs = 64 * nws
+    top = segs.bit_length() - 1
+    ds = range(1, top + 1) if depths is None else [d for d in depths if 1 <= d <= top]
+    return list(range(5)) + [3 | (d << 4) for d in ds]
+
+
+def ragged_sweep(prec, kmin=5, kmax=13, rows_per_class=1 << 23, reps=10, seed=3,
+                 pcr_depths=None):
     """Per size bucket: time every shape of the ragged kernel that holds the bucket's
-    largest n with every interface solver, on systems drawn log-uniformly inside the
-    bucket; keep the fastest whose answer matches the FP64 ragged solve.  Bucket
-    bounds are the powers of two 2^kmin..2^kmax and every shape capacity between
-    them, so each bucket ends at a capacity."""
+    largest n with every interface solver (and PCR + Thomas at every PCR depth), on
+    systems drawn log-uniformly inside the bucket; keep the fastest whose answer
+    matches the FP64 ragged solve.  Bucket bounds are the powers of two
+    2^kmin..2^kmax and every shape capacity between them, so each bucket ends at a
+    capacity."""
     import torch
     from .solver import RaggedPlan
     dtype = torch.float32 if prec == 0 else torch.float64
@@ -180,7 +197,7 @@ def ragged_sweep(prec, kmin=5, kmax=13, rows_per_class=1 << 23, reps=10, seed=3)
         for sh in shapes:
             if sh["cap"] < hi:
                 continue
-            for iface in range(5):
+            for iface in iface_codes(sh["NWS"], pcr_depths):
                 try:
                     plan = RaggedPlan(off, dtype=dtype, shape=sh["index"], iface=iface)
                 except _capi.TridiagError as e:
@@ -310,12 +327,18 @@ def main(argv=None):
     ap.add_argument("--table", default=None, help="write the winners as a tuning table")
     ap.add_argument("--ragged", action="store_true",
                     help="tune the ragged kernel per size bucket instead (R lines of the table)")
+    ap.add_argument("--pcr-depths", type=int, nargs="*", default=None,
+                    help="ragged: PCR depths of the PCR + Thomas family to time (default: every "
+                         "depth a shape allows)")
+    ap.add_argument("--kmin", type=int, default=5, help="ragged: smallest bucket 2^kmin")
+    ap.add_argument("--kmax", type=int, default=13, help="ragged: largest bucket 2^kmax")
     args = ap.parse_args(argv)
     if args.ragged:
         t0 = time.time()
         buckets = []
         for prec in args.prec:
-            buckets += ragged_sweep(prec, reps=args.reps)
+            buckets += ragged_sweep(prec, kmin=args.kmin, kmax=args.kmax, reps=args.reps,
+                                    pcr_depths=args.pcr_depths)
         if args.out:
             with open(args.out, "w") as f:
                 json.dump(buckets, f, indent=1)

@@ -290,7 +290,7 @@ hipError_t launch_adj(const AdjParams<T> &p, hipStream_t st) {
     if (tiles > 0x7fffffff) return hipErrorInvalidValue;
     auto kern = tile_adj_kernel<T, M, NW, S, IL, MODE>;
     size_t dyn = 0;
-    if (p.iface >= kIfacePCR) dyn = (size_t)iface_lds_bytes<T>(64 * NW);
+    if (iface_family(p.iface) >= kIfacePCR) dyn = (size_t)iface_lds_bytes<T>(64 * NW);
     else if (p.xchg_lds) dyn = (size_t)6 * 64 * NW * sizeof(T);
     if (dyn) {
         static size_t granted = 0;

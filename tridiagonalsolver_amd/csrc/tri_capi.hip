@@ -8,9 +8,14 @@
 
 #include <dlfcn.h>
 
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <mutex>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -167,7 +172,7 @@ template <typename T> static long lds_need_iface(const TileEntry<T> &e, int ifac
     const long nws = e.IL ? e.NW : e.NW / e.S;
     long st = (e.ALDS ? (long)(e.M - 2) * lanes * w : 0) + (e.XP ? lanes * e.M * w : 0) +
               (long)e.S * 2 * nws * 8 * w;
-    long dyn = iface >= kIfacePCR ? 8 * lanes * w : (xchg_lds ? 6 * lanes * w : 0);
+    long dyn = iface_family(iface) >= kIfacePCR ? 8 * lanes * w : (xchg_lds ? 6 * lanes * w : 0);
     return st + dyn;
 }
 template <typename T> static long lds_need(const TileEntry<T> &e, int algo) {
@@ -401,6 +406,23 @@ static int32_t coop_spin_limit() {
     return v;
 }
 
+// Tag of the next cooperative solve: a 64-bit sequence from a random start (time,
+// process and a random device), never the same for two solves of the process, and
+// with a different start in every process, so that the flag words a solve finds
+// in its scratch (left by earlier solves or never written) do not carry its tag.
+static uint64_t coop_next_tag() {
+    static std::atomic<uint64_t> next{[] {
+        std::random_device rd;
+        uint64_t s = ((uint64_t)rd() << 32) ^ rd();
+        s ^= (uint64_t)std::chrono::steady_clock::now().time_since_epoch().count() * 0x9E3779B97F4A7C15ull;
+        s ^= (uint64_t)getpid() << 17;
+        return s | 1;
+    }()};
+    uint64_t t = next.fetch_add(1, std::memory_order_relaxed);
+    if (t == 0) t = next.fetch_add(1, std::memory_order_relaxed);   // keep 0 (fresh memory) out
+    return t;
+}
+
 // Which cooperative kernel runs: the entry's choice, or $TRIDIAG_COOP_PIPE (1: the
 // one that waits one task later, 0: the one that waits before its next task).
 static int32_t coop_pipelined(bool entry_default) {
@@ -486,7 +508,7 @@ template <typename T> struct SolvePlan {
     const CoopEntry<T> *coop = nullptr;
     int K = 0;                 // cooperative: row blocks per system
     int grid = 0;              // cooperative: persistent workgroups
-    size_t forms_bytes = 0;    // cooperative: scratch for the block forms, then counters
+    size_t forms_bytes = 0;    // cooperative: scratch for the block forms, then flags
     size_t scratch_bytes = 0;
 };
 
@@ -515,7 +537,7 @@ static bool coop_plan(int algo, int layout, int64_t n, int64_t batch, size_t fix
     if (cap < K) return false;
     const int64_t P = std::min<int64_t>(ntask, (cap / K) * (int64_t)K);
     const size_t forms = (((size_t)ntask * e->S * 6 * sizeof(T)) + 255) & ~(size_t)255;
-    const size_t bytes = forms + (size_t)ngroups * kCoopCtlStride * sizeof(int32_t);
+    const size_t bytes = forms + (size_t)ngroups * kCoopFlagStride * sizeof(uint64_t);
     if (fixed_bytes && bytes > fixed_bytes) return false;
     pl.kind = kPlanCoop;
     pl.coop = e;
@@ -543,11 +565,11 @@ static int coop_run(const SolvePlan<T> &pl, int layout, int64_t n, int64_t batch
                     (int32_t)(il ? n * batch * (int64_t)sizeof(T) : 0), (int32_t)pl.K,
                     (int32_t)ngroups, coop_spin_limit(), coop_pipelined(e.pipe),
                     reinterpret_cast<T *>(scratch),
-                    reinterpret_cast<int32_t *>(reinterpret_cast<char *>(scratch) +
-                                                pl.forms_bytes)};
+                    reinterpret_cast<uint64_t *>(reinterpret_cast<char *>(scratch) +
+                                                 pl.forms_bytes),
+                    coop_next_tag()};
     int rc = TRIDIAG_OK;
-    hipError_t err = launch_coop_reset(p.ctl, (int32_t)(ngroups * kCoopCtlStride), st);
-    if (err == hipSuccess) err = e.launch(p, pl.grid, st);
+    const hipError_t err = e.launch(p, pl.grid, st);
     if (err != hipSuccess) rc = hip_fail(err, "cooperative solve launch");
     if (!fx) {
         const hipError_t ef = hipFreeAsync(scratch, st);
@@ -779,12 +801,12 @@ static int bulk_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b,
 // ---------------------------------------------------------------- no-allocation solve
 // Scratch of tridiag_solve_noalloc: the chunked solve's per-chunk outer forms and
 // boundary values (8 words per chunk of every system of one launch group), or the
-// cooperative solve's block forms and group counters.  One copy per GPU, at a
+// cooperative solve's block forms and publication flags.  One copy per GPU, at a
 // device address of its own, held until the process ends.  Every
 // tridiag_solve_noalloc call on a GPU shares that GPU's copy; calls that overlap
 // on one GPU race.  The chunked solve writes every word it reads; the cooperative
-// solve clears its counters with a kernel before each use, so the contents left
-// by an earlier call need no clearing.
+// solve's flags only count when they carry the solve's own tag, which no earlier
+// call used, so the contents left by an earlier call need no clearing.
 constexpr size_t kNoAllocScratchBytes = TRIDIAG_NOALLOC_SCRATCH_BYTES;
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
@@ -897,8 +919,10 @@ static int solve_tile_impl(int index, int iface, int layout, int64_t n, int64_t 
                            T *b, T *c, T *d, hipStream_t st) {
     TileTable<T> tab = tile_table<T>();
     if (index < 0 || index >= tab.count) return fail(TRIDIAG_EINVAL, "tile index out of range");
-    if (iface < kIfaceWM || iface > kIfaceCR) return fail(TRIDIAG_EINVAL, "interface solver id out of range");
     const TileEntry<T> &e = tab.entries[index];
+    if (!iface_code_valid(iface, e.cap / e.M))
+        return fail(TRIDIAG_EINVAL, "interface solver code out of range (family 0-4; a PCR depth "
+                                    "only for family 3, at most log2 of the tile's segments)");
     if (lds_need_iface(e, iface, 0) > max_lds_per_block())
         return fail(TRIDIAG_EUNSUPPORTED, "the interface solver's LDS does not fit beside this tile");
     if (n < 1 || batch < 0 || layout < 0 || layout > 2) return fail(TRIDIAG_EINVAL, "bad shape");
@@ -935,7 +959,7 @@ static void ragged_choice(int prec, int64_t n, int &shape, int &iface) {
     for (const RaggedTuneRow &r : g_rtune) {
         if (r.prec != prec || n > r.nmax) continue;
         if (r.shape < 0 || r.shape >= kRaggedShapes || ragged_cap(r.shape) < n) continue;
-        if (r.iface < kIfaceWM || r.iface > kIfaceCR) continue;
+        if (!iface_code_valid(r.iface, ragged_segments(r.shape))) continue;
         shape = r.shape;
         iface = r.iface;
         return;

@@ -6,18 +6,6 @@
 
 namespace tri {
 
-__global__ void coop_reset_kernel(int32_t *ctl, int32_t count) {
-    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
-        ctl[i] = 0;
-}
-
-hipError_t launch_coop_reset(int32_t *ctl, int32_t count, hipStream_t st) {
-    if (count <= 0) return hipSuccess;
-    const int blocks = std::min<int32_t>(64, (count + 255) / 256);
-    hipLaunchKernelGGL(coop_reset_kernel, dim3(blocks), dim3(256), 0, st, ctl, count);
-    return hipGetLastError();
-}
-
 template <typename T, int M, int NW, int S, bool IL, int KMAX>
 static hipError_t launch_coop(const CoopParams<T> &p, int grid, hipStream_t st) {
     if (grid < 1 || p.K < 1 || p.K > KMAX || grid % p.K != 0) return hipErrorInvalidValue;
@@ -69,12 +57,12 @@ template <typename T, int M, int NW, int S, bool IL, int KMAX> static int coop_m
 // the fastest shapes; 4-wave interleaved workgroups (2 per CU, 32 blocks of 128
 // rows at n = 4096) and 8-wave contiguous ones were slower at every size.
 // PIPE: the entry runs coop_pipe_kernel (waits one task later).  Measured (ms,
-// pipelined / waiting): FP64 interleaved 4096 x 4096 0.226 / 0.237 and 8192 x 2048
-// 0.265 / 0.262; FP32 interleaved 4096 x 4096 0.116 / 0.117, 8192 x 2048 0.131 /
-// 0.135; contiguous FP32 20000 rows 0.099 / 0.090, FP64 0.193 / 0.182 (the
-// pipelined contiguous kernels spill).
+// pipelined / waiting): FP64 interleaved 4096 x 4096 0.226 / 0.233; FP32
+// interleaved 4096 x 4096 0.111 / 0.113, 8192 x 2048 0.129 / 0.132; contiguous
+// FP32 20000 rows 0.098 / 0.083, FP64 0.190 / 0.178, FP32 65536 rows 0.100 / 0.086
+// (the pipelined contiguous kernels spill).
 static const CoopEntry<float> kCoopF32[] = {
-    TRI_COOP_ENTRY(float, 16, 8, 32, true, 32, false),
+    TRI_COOP_ENTRY(float, 16, 8, 32, true, 32, true),
     TRI_COOP_ENTRY(float, 16, 4, 1, false, 16, false),
 };
 static const CoopEntry<double> kCoopF64[] = {

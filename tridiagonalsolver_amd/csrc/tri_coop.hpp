@@ -32,10 +32,12 @@
 // Publication without L2 writeback or invalidate: the block forms are written
 // with agent-scope atomic stores (which write through to the level where all
 // XCDs meet), the writing lanes wait until those stores have completed
-// (s_waitcnt) before the workgroup barrier, and one lane then bumps the group's
-// arrival counter with an agent-scope atomic add.  The waiting side polls the
-// counter with agent-scope atomic loads and reads the forms with agent-scope
-// atomic loads too, so no stale copy in an L1 or L2 can serve them.  An
+// (s_waitcnt) before the workgroup barrier, and one lane then stores the solve's
+// tag into the block's flag word with an agent-scope atomic store.  The waiting
+// side polls the group's flag words with agent-scope atomic loads and reads the
+// forms with agent-scope atomic loads too, so no stale copy in an L1 or L2 can
+// serve them.  Tagged flags instead of an arrival counter: nothing has to be
+// cleared before a solve (a counter needed a reset kernel per solve, 2-6 us).  An
 // acquire/release pair at agent scope would also order them, but it compiles to
 // an L2 writeback (release) and invalidate (acquire) per workgroup and task,
 // which the XCDs' L2s serve one at a time: measured on MI355X, FP32 interleaved
@@ -62,12 +64,17 @@ template <typename T> struct CoopParams {
     int32_t spin_limit;  // polls of a wait before it gives up
     int32_t pipelined;   // 1: coop_pipe_kernel (wait one task later), 0: coop_kernel
     T *forms;            // [ngroups][K][S][6] block outer forms
-    int32_t *ctl;        // [kCoopCtlStride * g] blocks of group g published
+    uint64_t *flags;     // [kCoopFlagStride * g + j]: the tag once block j of group g is published
+    uint64_t tag;        // this solve's tag: different from every earlier solve's
 };
 
-// Group counters sit 256 bytes apart: the polls of different groups then go to
-// different memory channels instead of all to one line.
-constexpr int kCoopCtlStride = 64;
+// Publication flags: one 8-byte word per block of a group, the group's words in a
+// 256-byte line of their own (so that the polls of different groups go to
+// different memory channels).  A block is published when its word holds the
+// solve's tag.  Tags never repeat (a 64-bit sequence from a random start), so
+// words left by earlier solves, or never written, do not read as published and
+// the flags need no clearing between solves.
+constexpr int kCoopFlagStride = 32;
 // s_sleep argument between two polls of a wait (64 clocks per unit).
 #ifndef TRI_COOP_POLL_SLEEP
 #define TRI_COOP_POLL_SLEEP 8
@@ -77,6 +84,21 @@ template <typename T, int M, int NW, int S, bool IL> struct CoopShape {
     static constexpr int SW = IL ? 64 / S : 64;      // segments of a system per wave
     static constexpr int R = SW * NW * M;            // rows per block
 };
+
+// Wave 0 of a workgroup waits until all K blocks of group g carry the solve's tag:
+// lane k polls block k's word, and the wave goes on once every lane has seen it
+// (one vote per poll).  Returns 0 if spin_limit polls pass first.
+template <typename T>
+__device__ __forceinline__ int coop_wait(const CoopParams<T> &p, int g, int lane) {
+    const uint64_t *f = p.flags + kCoopFlagStride * g;
+    for (int polls = 0;; ++polls) {
+        const bool seen =
+            lane >= p.K || __hip_atomic_load(f + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == p.tag;
+        if (__all(seen)) return 1;
+        if (polls >= p.spin_limit) return 0;
+        __builtin_amdgcn_s_sleep(TRI_COOP_POLL_SLEEP);
+    }
+}
 
 // Folds wave-block outer forms w = 0..nw-1 (first row at [w][0..2], last row at
 // [w][3..5]) left to right into the outer form of the whole block.
@@ -237,6 +259,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)
     static_assert(IL || S == 1, "contiguous groups hold one system");
     static_assert(IL ? (S * (int)sizeof(T)) % 16 == 0 : M % (16 / (int)sizeof(T)) == 0,
                   "blocks move whole 16-byte chunks");
+    static_assert(KMAX <= kCoopFlagStride, "a group's flags fill at most one line");
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int sl = IL ? lane % S : 0;        // system of the lane inside the group
     const int kw = IL ? lane / S : lane;     // segment inside the wave-block
@@ -309,8 +332,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)
         }
         __syncthreads();
         if (tid == 0)
-            __hip_atomic_fetch_add(p.ctl + kCoopCtlStride * gi, 1, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(p.flags + kCoopFlagStride * gi + j, p.tag, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         // the next round's block streams in while this group completes
         const int next = task + P;
         if (next < ntask) {
@@ -321,15 +344,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)
                                                                 p.rstride, (int64_t)g1 * S, lane, g);
             else contig_block_issue<T, M, true>(rr, wrow, lane, g);
         }
-        if (tid == 0) {
-            int ok = 1;
-            for (int polls = 0; __hip_atomic_load(p.ctl + kCoopCtlStride * gi, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT) < K;) {
-                if (++polls > p.spin_limit) { ok = 0; break; }
-                __builtin_amdgcn_s_sleep(TRI_COOP_POLL_SLEEP);
-            }
-            s_ok = ok;
-        }
+        if (tid < 64) s_ok = coop_wait(p, gi, tid);
         __syncthreads();
         const bool ok = s_ok != 0;
         // the group's K block forms, staged in LDS by the whole workgroup
@@ -389,14 +404,9 @@ __device__ __forceinline__ void coop_finish(const CoopParams<T> &p, int gi, int 
     const int64_t brow = (int64_t)j * R;
     const int64_t len = IL ? R : ((n - brow < R) ? n - brow : R);
     const T *fg = p.forms + (int64_t)gi * K * S * 6;
-    if (tid == 0) {
-        int ok = 1;
-        for (int polls = 0; __hip_atomic_load(p.ctl + kCoopCtlStride * gi, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT) < K;) {
-            if (++polls > p.spin_limit) { ok = 0; break; }
-            __builtin_amdgcn_s_sleep(TRI_COOP_POLL_SLEEP);
-        }
-        *s_ok = ok;
+    if (tid < 64) {
+        const int ok = coop_wait(p, gi, tid);
+        if (tid == 0) *s_ok = ok;
     }
     __syncthreads();
     const bool ok = *s_ok != 0;
@@ -529,8 +539,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)
         }
         __syncthreads();
         if (tid == 0)
-            __hip_atomic_fetch_add(p.ctl + kCoopCtlStride * gi, 1, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(p.flags + kCoopFlagStride * gi + j, p.tag, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         const int next = task + P;
         if (next < ntask) {
             const int g1 = next / K, j1 = next - (next / K) * K;
@@ -565,8 +575,6 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)
     }
 }
 
-// Clears the group arrival counters before a solve.
-__global__ void coop_reset_kernel(int32_t *ctl, int32_t count);
 
 template <typename T> struct CoopEntry {
     int M, NW, S;
@@ -583,6 +591,5 @@ template <typename T> struct CoopTable {
 };
 CoopTable<float> coop_table_f32();
 CoopTable<double> coop_table_f64();
-hipError_t launch_coop_reset(int32_t *ctl, int32_t count, hipStream_t st);
 
 }  // namespace tri

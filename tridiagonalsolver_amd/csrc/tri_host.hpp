@@ -32,7 +32,7 @@ hipError_t launch_tile(const TileParams<T> &p, hipStream_t st) {
     // LDS for the interface solvers: 2 rows x 4 coefficients per lane (PCR/CR), or
     // 6 outer-form values per lane (Wang-Mou merges exchanged through LDS).
     size_t dyn = 0;
-    if (p.iface >= kIfacePCR) dyn = (size_t)iface_lds_bytes<T>(64 * NW);
+    if (iface_family(p.iface) >= kIfacePCR) dyn = (size_t)iface_lds_bytes<T>(64 * NW);
     else if (p.xchg_lds) dyn = (size_t)6 * 64 * NW * sizeof(T);
     if (dyn) {
         static size_t granted = 0;      // per instantiation

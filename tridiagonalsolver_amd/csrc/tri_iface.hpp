@@ -19,6 +19,15 @@ enum Iface : int {
     kIfaceCR = 4,       // cyclic reduction
 };
 
+// An interface-solver code is the family (Iface) in bits 0-3 and, for
+// kIfacePCRThomas, the PCR depth in bits 4-7: the PCR steps taken before Thomas
+// runs along the remaining subsystems.  d steps leave 2^d interleaved subsystems
+// of the 2P interface rows (the first step pairs each segment's two rows, each
+// further one doubles the segment stride); depth 0 is the family's default, one
+// step per doubling up to the wave's segment stride.
+__host__ __device__ constexpr int iface_family(int code) { return code & 15; }
+__host__ __device__ constexpr int iface_pcr_depth(int code) { return (code >> 4) & 15; }
+
 template <typename T> struct Row4 {
     T a, b, c, d;
 };

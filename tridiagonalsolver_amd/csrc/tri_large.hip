@@ -7,7 +7,7 @@ template <typename T, int M, int NW>
 static hipError_t launch_large_shape(const LargeParams<T> &p, hipStream_t st) {
     if (p.K < 1 || p.K > 1024 || p.batch < 1 || p.batch > 65535) return hipErrorInvalidValue;
     size_t dyn = 0;
-    if (p.iface >= kIfacePCR) dyn = (size_t)iface_lds_bytes<T>(64 * NW);
+    if (iface_family(p.iface) >= kIfacePCR) dyn = (size_t)iface_lds_bytes<T>(64 * NW);
     auto solve = chunk_kernel<T, M, NW, kChunkSolve>;
     if (dyn) {
         static size_t granted = 0;

@@ -27,8 +27,22 @@ constexpr int kRaggedLanes = 512;
 constexpr int kRaggedWaves = kRaggedLanes / 64;
 constexpr int kRaggedMaxN = 64 * 8 * 16;
 constexpr int kRaggedMaxM = 16;
-// Interface solver ids of a bucket: 0 .. kRaggedIfaces-1 (the Iface values).
+// Interface solver families of a bucket: 0 .. kRaggedIfaces-1 (the Iface values).
 constexpr int kRaggedIfaces = 5;
+// Interface-solver codes: the family in bits 0-3 and, for the PCR + Thomas family
+// (3), the PCR depth d in bits 4-7 (0: the family's default).  Valid for a system
+// of P lane segments when d leaves a stride of at most P/2 segments: 1 <= d <=
+// log2(P).  Codes are below kIfaceCodes.
+constexpr int kIfaceCodes = 256;
+inline bool iface_code_valid(int code, int P) {
+    if (code < 0 || code >= kIfaceCodes) return false;
+    const int fam = code & 15, depth = code >> 4;
+    if (fam >= kRaggedIfaces) return false;
+    if (depth == 0) return true;
+    return fam == 3 && (1 << (depth - 1)) <= P / 2;
+}
+// Lane segments per system of ragged shape k.
+constexpr int ragged_segments(int k) { return 64 * kRaggedShapeTab[k].NWS; }
 
 // Work buffer: a 64-byte header, then one 128-byte record per item:
 //   r[0] shape, r[1] interface solver, r[2] number of systems, r[3] 0,
@@ -71,8 +85,8 @@ inline int ragged_default_shape(int64_t n) {
 
 // Builds the work-buffer image of n_sys systems (offsets off[0..n_sys]):
 // validates the offsets, buckets every non-empty system by (shape, interface
-// solver) — choice(n, shape, iface) gives a system's bucket unless force_shape /
-// force_iface >= 0 fix it — orders the buckets by decreasing capacity (then
+// solver code) — choice(n, shape, iface) gives a system's bucket unless
+// force_shape / force_iface >= 0 fix it — orders the buckets by decreasing capacity (then
 // increasing systems per item) and writes one record per item of up to S systems.
 // On success `image` holds the header and records (its size is the number of
 // bytes to upload) and the return value is 0; otherwise 1 (bad argument) or 2
@@ -84,10 +98,10 @@ int ragged_schedule(int prec, int64_t n_sys, const int64_t *off, int force_shape
     if (n_sys < 0 || n_sys > 0x7fffffff) return err = "n_sys out of range", 1;
     if (n_sys > 0 && !off) return err = "null offsets", 1;
     if (force_shape >= kRaggedShapes || force_shape < -1) return err = "shape index out of range", 1;
-    if (force_iface >= kRaggedIfaces || force_iface < -1)
+    if (force_iface != -1 && !iface_code_valid(force_iface, 64 * 8))
         return err = "interface solver id out of range", 1;
     if (n_sys > 0 && off[0] < 0) return err = "offsets[0] is negative", 1;
-    constexpr int NF = kRaggedIfaces;
+    constexpr int NF = kIfaceCodes;
     std::vector<int32_t> key((size_t)n_sys, -1);
     std::vector<int64_t> cnt((size_t)kRaggedShapes * NF, 0);
     for (int64_t s = 0; s < n_sys; ++s) {
@@ -101,14 +115,20 @@ int ragged_schedule(int prec, int64_t n_sys, const int64_t *off, int force_shape
         int shape = -1, iface = 0;
         choice(n, shape, iface);
         if (shape < 0 || shape >= kRaggedShapes || ragged_cap(shape) < n) shape = ragged_default_shape(n);
-        if (iface < 0 || iface >= NF) iface = 0;
         if (force_shape >= 0) {
             if (ragged_cap(force_shape) < n)
                 return err = "system " + std::to_string(s) + " exceeds the capacity of the forced shape",
                        1;
             shape = force_shape;
         }
-        if (force_iface >= 0) iface = force_iface;
+        if (!iface_code_valid(iface, ragged_segments(shape))) iface = 0;
+        if (force_iface >= 0) {
+            if (!iface_code_valid(force_iface, ragged_segments(shape)))
+                return err = "PCR depth " + std::to_string(force_iface >> 4) +
+                             " exceeds the segments of shape " + std::to_string(shape),
+                       1;
+            iface = force_iface;
+        }
         key[(size_t)s] = shape * NF + iface;
         ++cnt[(size_t)key[(size_t)s]];
     }

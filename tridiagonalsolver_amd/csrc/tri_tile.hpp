@@ -506,12 +506,16 @@ __device__ __forceinline__ void interface_solve(T aL, T DLr, T ELr, T FLr, T ARr
                                                 int iface, int xchg_lds, char *tri_dyn_lds,
                                                 T &xL, T &xR) {
     const int seg = ws * SW + kw;            // segment index inside the system
-    if (iface >= kIfacePCR) {
+    const int fam = iface_family(iface);
+    if (fam >= kIfacePCR) {
         // PCR / PCR + Thomas / CR on the 2P interface rows, through LDS
         IfaceLds<T> IL_ = iface_lds<T>(tri_dyn_lds, S, 2 * P);
         Row4<T> rl{aL, DLr, ELr, FLr}, rr{ARr, DRr, cR, FRr};
-        if (iface == kIfacePCR || iface == kIfacePCRThomas) {
-            const int lim = (iface == kIfacePCR) ? P : SW;
+        if (fam == kIfacePCR || fam == kIfacePCRThomas) {
+            // PCR stops at segment stride lim (depth d: 2^(d-1)); lim = P is full PCR
+            const int depth = iface_pcr_depth(iface);
+            int lim = (fam == kIfacePCR) ? P : (depth > 0 ? (1 << (depth - 1)) : SW);
+            if (lim > P / 2) lim = P;
             pcr_interface(IL_, sl, seg, P, lim, rl, rr);
             if (lim == P) {
                 xL = rl.d * rcp(rl.b);
@@ -520,7 +524,7 @@ __device__ __forceinline__ void interface_solve(T aL, T DLr, T ELr, T FLr, T ARr
                 IL_.put(sl, 2 * seg, rl);
                 IL_.put(sl, 2 * seg + 1, rr);
                 __syncthreads();
-                strided_thomas(IL_, S, P, SW, tid);
+                strided_thomas(IL_, S, P, lim, tid);
                 xL = IL_.d[sl * 2 * P + 2 * seg];
                 xR = IL_.d[sl * 2 * P + 2 * seg + 1];
             }
@@ -537,7 +541,7 @@ __device__ __forceinline__ void interface_solve(T aL, T DLr, T ELr, T FLr, T ARr
     // Wang-Mou: outer form of the segment, merges across lanes, pGE chain across blocks
     Outer<T> oL, oR;
     segment_outer(aL, DLr, ELr, FLr, ARr, DRr, cR, FRr, oL, oR);
-    const int half = iface == kIfaceWMHalf ? 1 : 0;
+    const int half = fam == kIfaceWMHalf ? 1 : 0;
     const int BS = SW >> half;                   // segments per merged block
     T *xs = reinterpret_cast<T *>(tri_dyn_lds) + (size_t)(tid & ~63) * 6;
     wm_wave_merges<T, SW, LS>(oL, oR, lane, sl, kw, BS, xchg_lds != 0, xs);
