@@ -1,9 +1,9 @@
 """GPU parity of the cooperative row-block solve (K workgroups per group of
 systems, block forms exchanged through global memory), through the C-ABI,
 against the FP64 oracle: normwise relative error <= 1e-5 (FP32) / 1e-12 (FP64)
-over every system.  Interleaved systems take it from 2048 rows (FP32) / 4096
-rows (FP64) when 256-row blocks divide n, contiguous systems above one
-workgroup's rows in up to 16 blocks; every case asserts the plan it tests.  A
+over every system.  Interleaved systems take it from 2048 rows when 256-row
+blocks divide n, contiguous systems above one
+workgroup's rows in up to 32 blocks; every case asserts the plan it tests.  A
 forced zero spin limit (run in a child process) checks that a wait that gives up
 ends the kernel and marks its block with NaN, never with a wrong finite value."""
 import os
@@ -63,7 +63,9 @@ def test_interleaved_low_margin(dtype):
 
 @pytest.mark.parametrize("dtype,n,batch,K", [
     (np.float32, 20000, 7, 5), (np.float32, 65536, 12, 16), (np.float32, 8193 * 3, 5, 7),
-    (np.float64, 20000, 7, 10), (np.float64, 32768, 9, 16), (np.float64, 8193 * 3, 5, 13)])
+    (np.float32, 131072, 3, 32), (np.float32, 4096 * 31 + 17, 4, 32),
+    (np.float64, 20000, 7, 10), (np.float64, 32768, 9, 16), (np.float64, 8193 * 3, 5, 13),
+    (np.float64, 65536, 5, 32), (np.float64, 40000, 6, 20)])
 def test_contiguous(dtype, n, batch, K):
     _assert_coop(0 if dtype == np.float32 else 1, 0, n, batch, K)
     announce("coop contiguous", np.dtype(dtype).name, n, batch)
@@ -77,11 +79,15 @@ def test_plan_boundaries():
     """Where the planner switches to and from the cooperative solve."""
     import tridiagonalsolver_amd as T
     assert T.plan(0, -1, 2, 1024, 64)["kind"] == "tile"
-    assert T.plan(1, -1, 2, 2048, 64)["kind"] == "tile"
+    assert T.plan(1, -1, 2, 1024, 64)["kind"] == "tile"
+    assert T.plan(1, -1, 2, 2048, 64)["kind"] == "cooperative"
     assert T.plan(0, -1, 2, 2304, 64)["kind"] == "cooperative"
     assert T.plan(0, -1, 2, 2100, 64)["kind"] == "tile"             # blocks must divide n
     assert T.plan(0, -1, 2, 8448, 64)["kind"] == "via_contiguous"   # 33 blocks
-    assert T.plan(1, -1, 0, 40000, 8)["kind"] == "chunked"          # 20 blocks
+    assert T.plan(1, -1, 0, 65536, 8)["kind"] == "cooperative"      # 32 blocks
+    assert T.plan(1, -1, 0, 65537, 8)["kind"] == "chunked"          # 33 blocks
+    assert T.plan(0, -1, 0, 131072, 8)["kind"] == "cooperative"
+    assert T.plan(0, -1, 0, 131073, 8)["kind"] == "chunked"
     assert T.plan(0, 5, 0, 20000, 8)["kind"] == "chunked"           # PCR interface family
     assert T.plan(0, 5, 2, 4096, 8)["kind"] == "tile"
 
@@ -145,7 +151,10 @@ from gpu_util import TOL, check_against_oracle, make_case, run_solve
 for dtype, layout, n, batch in ((np.float32, 2, 4096, 96), (np.float64, 2, 4096, 48),
                                 (np.float32, 2, 8192, 40), (np.float64, 2, 8192, 17),
                                 (np.float32, 0, 20000, 7), (np.float64, 0, 20001, 5),
-                                (np.float32, 0, 65536, 3)):
+                                (np.float32, 0, 65536, 3),
+                                # several rounds of the persistent grid
+                                (np.float32, 2, 4096, 4096), (np.float32, 0, 32768, 200),
+                                (np.float64, 0, 16384, 300)):
     prec = 0 if dtype == np.float32 else 1
     assert T.plan(prec, -1, layout, n, batch)["kind"] == "cooperative"
     arrs = make_case(n, batch, layout, dtype, seed=n + batch)

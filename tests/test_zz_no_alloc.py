@@ -40,7 +40,7 @@ def _run(capi, fn, prec, algo, layout, n, batch, arrs):
     (0, 2, -1, [(4096, 640), (2048, 40)]),    # FP32 interleaved, cooperative blocks
     (1, 2, -1, [(4096, 320), (1024, 9)]),     # FP64 interleaved
     (1, 0, 5, [(24581, 40), (9000, 5)]),      # PCR interface: the chunked solve
-    (1, 0, -1, [(70000, 6), (40000, 3)]),     # FP64 beyond 16 blocks: the chunked solve
+    (1, 0, -1, [(70000, 6), (66000, 3)]),     # FP64 beyond 32 blocks: the chunked solve
 ])
 def test_scratch_paths_two_calls(lib, prec, layout, algo, shapes):
     dtype = np.float32 if prec == 0 else np.float64

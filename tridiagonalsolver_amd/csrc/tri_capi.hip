@@ -389,15 +389,16 @@ static int64_t env_int(const char *name, int64_t dflt) {
 // Interleaved systems of at least this many rows take the cooperative solve
 // ($TRIDIAG_COOP_MIN_ROWS overrides for both precisions; a negative value turns
 // the cooperative solve off for both layouts).  Measured on MI355X at 2^24
-// unknowns against the one-workgroup tiles: FP32 1024 rows 0.104 vs 0.093 ms,
-// 2048 rows 0.108 vs 0.122 ms, 4096 rows 0.117 vs 0.118 ms; FP64 2048 rows 0.215
-// vs 0.207 ms, 4096 rows 0.232 vs 0.253 ms.  Above 4096 rows (no interleaved
-// tile) it replaces the solve through a contiguous copy: 8192 rows 0.138 vs
-// 0.201 ms (FP32), 0.257 vs 0.406 ms (FP64).
+// unknowns against the one-workgroup tiles: FP32 1024 rows 0.092 vs 0.091 ms,
+// 1536 rows 0.164 vs 0.146 ms, 2048 rows 0.091 vs 0.122 ms, 4096 rows 0.107 vs
+// 0.118 ms; FP64 2048 rows 0.204 vs 0.213 ms, 3072 rows 0.327 vs 0.358 ms, 4096
+// rows 0.214-0.221 vs 0.253 ms.  Above 4096 rows (no interleaved tile) it
+// replaces the solve through a contiguous copy: 8192 rows 0.123 vs 0.201 ms
+// (FP32), 0.257 vs 0.406 ms (FP64).
 static int64_t coop_min_rows_il(int prec) {
     static const int64_t env = env_int("TRIDIAG_COOP_MIN_ROWS", 0);
     if (env != 0) return env;
-    return prec == TRIDIAG_SINGLE ? 2048 : 4096;
+    return 2048;
 }
 // Polls of a cooperative wait before it gives up ($TRIDIAG_COOP_SPIN).
 static int32_t coop_spin_limit() {
@@ -432,15 +433,17 @@ static int32_t coop_pipelined(bool entry_default) {
 
 // The entry of the cooperative table for (layout, n): interleaved blocks must
 // divide n; K = blocks per system in [2, kmax].  Contiguous systems take it up to
-// 16 blocks: measured on MI355X at 2^24 unknowns against the chunked solve, FP32
-// 20000 rows 0.090 vs 0.114 ms and 65536 rows 0.094 vs 0.110 ms, FP64 20000 rows
-// 0.184 vs 0.212 ms; with more blocks every group waits on more workgroups and it
-// fell behind (FP64 65536 rows in 32 blocks 0.220 vs 0.208 ms, 2^20 rows 1.6 vs
-// 0.21 ms).
+// 32 blocks (FP32 131072 rows, FP64 65536): measured on MI355X at 2^24 unknowns
+// against the chunked solve, FP32 20000 rows 0.076 vs 0.113 ms and 131072 rows
+// 0.088 vs 0.107 ms, FP64 20000 rows 0.147 vs 0.209 ms and 65536 rows 0.173 vs
+// 0.206 ms.  Far beyond (2^20 rows, 512 blocks) every group would wait on hundreds
+// of workgroups: an early version ran 1.6 ms against the chunked 0.21 ms.
 template <typename T>
 static const CoopEntry<T> *choose_coop(bool il, int64_t n, int &K) {
     CoopTable<T> tab = coop_table<T>();
+    static const int64_t forced = env_int("TRIDIAG_COOP_ENTRY", -1);   // experiments
     for (int i = 0; i < tab.count; ++i) {
+        if (forced >= 0 && i != forced) continue;
         const CoopEntry<T> &e = tab.entries[i];
         if (e.IL != il) continue;
         const int64_t k = il ? (n % e.R == 0 ? n / e.R : 0) : (n + e.R - 1) / e.R;
@@ -533,7 +536,7 @@ static bool coop_plan(int algo, int layout, int64_t n, int64_t batch, size_t fix
     const int64_t ntask = ngroups * K;
     if (ntask > 0x7fffffff) return false;
     // every task of a round must be resident: P workgroups, a multiple of K
-    const int cap = e->max_grid();
+    const int cap = e->max_grid(coop_pipelined(e->pipe) != 0);
     if (cap < K) return false;
     const int64_t P = std::min<int64_t>(ntask, (cap / K) * (int64_t)K);
     const size_t forms = (((size_t)ntask * e->S * 6 * sizeof(T)) + 255) & ~(size_t)255;

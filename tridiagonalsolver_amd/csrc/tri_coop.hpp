@@ -249,8 +249,10 @@ __device__ __forceinline__ void coop_rsrc(const CoopParams<T> &p, int g, int64_t
     }
 }
 
-template <typename T, int M, int NW, int S, bool IL, int KMAX>
-__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)))
+// PF: the next task's block is loaded into registers while this one finishes;
+// OCC: waves per SIMD the kernel is built for (its register budget).
+template <typename T, int M, int NW, int S, bool IL, int KMAX, bool PF, int OCC>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(OCC)))
     coop_kernel(CoopParams<T> p) {
     using Sh = TileShape<T, M, NW, S, IL, !IL, false, false, !IL, true, true>;
     constexpr int SW = IL ? 64 / S : 64;     // segments of a system per wave
@@ -277,17 +279,17 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)
     const int64_t n = p.n;
     int task = (int)blockIdx.x;
     u32x4 g[4][NQ];
-    {
-        const int g0 = task / K, j0 = task - (task / K) * K;
+    auto issue = [&](int t) {
+        const int g0 = t / K, j0 = t - (t / K) * K;
         __amdgpu_buffer_rsrc_t rr[4];
         coop_rsrc<T, IL>(p, g0, (int64_t)j0 * R, rr);
-        if (task < ntask) {
-            if constexpr (IL) il_block_issue<T, M, SW, S, true>(rr, (int64_t)j0 * R + wrow,
-                                                                p.rstride, (int64_t)g0 * S, lane, g);
-            else contig_block_issue<T, M, true>(rr, wrow, lane, g);
-        }
-    }
+        if constexpr (IL) il_block_issue<T, M, SW, S, true>(rr, (int64_t)j0 * R + wrow, p.rstride,
+                                                            (int64_t)g0 * S, lane, g);
+        else contig_block_issue<T, M, true>(rr, wrow, lane, g);
+    };
+    if (PF && task < ntask) issue(task);
     for (; task < ntask; task += P) {
+        if constexpr (!PF) issue(task);
         const int gi = task / K, j = task - gi * K;
         const int64_t brow = (int64_t)j * R;               // first row of the block
         const int64_t row0 = brow + wrow + (int64_t)kw * M;   // first row of the lane
@@ -335,16 +337,11 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)
             __hip_atomic_store(p.flags + kCoopFlagStride * gi + j, p.tag, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         // the next round's block streams in while this group completes
-        const int next = task + P;
-        if (next < ntask) {
-            const int g1 = next / K, j1 = next - (next / K) * K;
-            __amdgpu_buffer_rsrc_t rr[4];
-            coop_rsrc<T, IL>(p, g1, (int64_t)j1 * R, rr);
-            if constexpr (IL) il_block_issue<T, M, SW, S, true>(rr, (int64_t)j1 * R + wrow,
-                                                                p.rstride, (int64_t)g1 * S, lane, g);
-            else contig_block_issue<T, M, true>(rr, wrow, lane, g);
+        if (PF && task + P < ntask) issue(task + P);
+        if (tid < 64) {
+            const int okw = coop_wait(p, gi, tid);
+            if (tid == 0) s_ok = okw;
         }
-        if (tid < 64) s_ok = coop_wait(p, gi, tid);
         __syncthreads();
         const bool ok = s_ok != 0;
         // the group's K block forms, staged in LDS by the whole workgroup
@@ -583,7 +580,7 @@ template <typename T> struct CoopEntry {
     int kmax;    // most blocks per system (LDS for the block chain)
     bool pipe;   // coop_pipe_kernel by default
     hipError_t (*launch)(const CoopParams<T> &, int grid, hipStream_t);
-    int (*max_grid)();   // co-resident workgroups on the current device (0: unknown)
+    int (*max_grid)(bool pipelined);   // co-resident workgroups of the kernel (0: unknown)
 };
 template <typename T> struct CoopTable {
     const CoopEntry<T> *entries;
