@@ -827,9 +827,8 @@ def run_n1024(torch, T, steps, warmup, with_cpu, budget_s=2.0):
 
 def run_large(torch, T, steps, warmup):
     """Large single systems, FP64, dominance margin U[0.05, 0.2]: n = 2^20 x 16 and
-    2^16 x 256 systems (contiguous layout), solved by the automatic plan (the chunked
-    three-pass solve, or the cooperative row-block solve where its blocks fit), and
-    by the chunked solve alone for comparison."""
+    2^16 x 256 systems (contiguous layout), solved by the automatic plan: the chunked
+    three-pass solve, or the cooperative row-block solve where its blocks fit."""
     import numpy as np
     out = {}
     for n, batch in ((1 << 20, 16), (1 << 16, 256)):
@@ -843,8 +842,6 @@ def run_large(torch, T, steps, warmup):
             del a, b, c, d
         log("time", name)
         wall, kern = time_solver(torch, T, sets, n, batch, steps, warmup, -1, 0)
-        # the chunked solve alone: the PCR + Thomas id takes it at these sizes
-        cwall, ckern = time_solver(torch, T, sets, n, batch, steps, warmup, 5, 0)
         del sets
         torch.cuda.empty_cache()
         par = check_parity_lowmargin(torch, T, n, batch)
@@ -856,9 +853,7 @@ def run_large(torch, T, steps, warmup):
                      "hbm_frac": achieved / HBM_PEAK_GBS, "parity": par, "plan": pl,
                      "path": {"chunked": "chunked: reduce / chunk-interface merge / solve",
                               "cooperative": "cooperative row blocks (5 words per unknown)"}
-                     .get(pl["kind"], pl["kind"]),
-                     "chunked_only": {"plan": T.plan(1, 5, 0, n, batch), "kernel_ms": ckern,
-                                      "hbm_frac": 5 * 8 * U / (ckern * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+                     .get(pl["kind"], pl["kind"])}
         log(name, json.dumps(out[name]))
     return out
 
