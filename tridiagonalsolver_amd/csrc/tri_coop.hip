@@ -56,7 +56,8 @@ static int coop_max_grid(bool pipelined) {
 //                no PF, 2 per CU (OCC 4) 0.108.  FP32 2048 x 8192: 0.102 / 0.091
 //                (pipelined / OCC 4); 8192 x 2048: 0.129 / 0.123.  FP64 4096 x
 //                4096: 0.233 / 0.223 / 0.214 (PF / pipelined / OCC 4).  Blocks of
-//                128 rows (M = 8 FP32, 4 FP64) with K = 32: 0.148 / 0.282.
+//                128 rows (M = 8 FP32, 4 FP64) with K = 32: 0.148 / 0.282; 16-wave
+//                workgroups of M = 8 / 4 (no spills, one per CU): 0.133 / 0.286.
 //   contiguous   FP32 20000 rows: PF OCC 2 0.083, OCC 4 0.082, OCC 3 0.076;
 //                131072 rows (32 blocks): OCC 4 0.093, OCC 3 0.088, chunked 0.107.
 //                FP64 20000 rows: PF OCC 2 0.180, OCC 4 0.147, OCC 3 0.151, chunked
