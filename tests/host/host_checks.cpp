@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "../../tridiagonalsolver_amd/csrc/tri_coop_sched.hpp"
 #include "../../tridiagonalsolver_amd/csrc/tri_ragged_plan.hpp"
 
 using namespace tri;
@@ -180,9 +181,36 @@ static void layout_cases() {
     CHECK(g.rows == ((int64_t)1 << 20) && g.cols == ((int64_t)1 << 22) && g.transpose, "large");
 }
 
+// The cooperative schedule's slot -> group map: a bijection on [0, G) for every
+// round width W, identity for W <= 1, and within a round the groups lie at least
+// floor(G/W) apart once there is more than one round.
+static void coop_spread_cases() {
+    for (int G = 1; G <= 300; ++G)
+        for (int W = 0; W <= 70; ++W) {
+            std::vector<int> seen(G, 0);
+            for (int s = 0; s < G; ++s) {
+                const int g = coop_group(s, G, W);
+                CHECK(g >= 0 && g < G, "G=%d W=%d s=%d -> %d", G, W, s, g);
+                if (g >= 0 && g < G) ++seen[g];
+                if (W <= 1 || G <= W) CHECK(g == s, "identity G=%d W=%d s=%d", G, W, s);
+            }
+            for (int g = 0; g < G; ++g) CHECK(seen[g] == 1, "G=%d W=%d group %d x%d", G, W, g, seen[g]);
+            if (W > 1 && G > W)
+                for (int s = 0; s + 1 < G; ++s)
+                    if ((s + 1) % W != 0) {
+                        const int d = coop_group(s + 1, G, W) - coop_group(s, G, W);
+                        CHECK(d >= G / W, "G=%d W=%d slots %d,%d only %d apart", G, W, s, s + 1, d);
+                    }
+        }
+    // the measured case: 128 groups of 32 FP32 systems (4096 x 4096), 32 per round
+    CHECK(coop_group(0, 128, 32) == 0 && coop_group(1, 128, 32) == 4 && coop_group(32, 128, 32) == 1,
+          "4096 x 4096 spread");
+}
+
 int main() {
     ragged_cases();
     layout_cases();
+    coop_spread_cases();
     if (failures) {
         fprintf(stderr, "%d failures\n", failures);
         return 1;

@@ -176,3 +176,46 @@ def test_both_cooperative_kernels(pipe):
     r = subprocess.run([sys.executable, "-c", _FORCED_KERNEL % {"root": ROOT}], env=env,
                        capture_output=True, text=True, timeout=180)
     assert r.returncode == 0 and "ok" in r.stdout, r.stdout + r.stderr
+
+
+_NO_SPREAD = r"""
+import sys, numpy as np
+sys.path.insert(0, %(root)r)
+sys.path.insert(0, %(root)r + '/tests')
+from gpu_util import make_case, run_solve
+out = {}
+for i, (dtype, n, batch) in enumerate(%(cases)r):
+    dt = np.dtype(dtype).type
+    arrs = make_case(n, batch, 2, dt, seed=i + 11)
+    out["x%%d" %% i] = run_solve(arrs, n, batch, 2, -1, dt)
+np.savez(%(path)r, **out)
+print("ok", flush=True)
+"""
+_SPREAD_CASES = [("float32", 4096, 4096), ("float64", 4096, 1000), ("float32", 2048, 3000),
+                 ("float64", 2048, 16)]
+
+
+def test_group_spread_same_bits(tmp_path):
+    """The schedule spreads the groups of one round over the whole batch
+    (TRIDIAG_COOP_SPREAD, default on).  Each group's arithmetic is unchanged, so
+    x must be bit-identical to the run with neighbouring groups (child process:
+    the setting is read once).  Cases with one partial table column and with a
+    single round."""
+    import tridiagonalsolver_amd as T
+    path = str(tmp_path / "nospread.npz")
+    env = dict(os.environ, TRIDIAG_COOP_SPREAD="0")
+    announce("coop spread off (child process)")
+    r = subprocess.run([sys.executable, "-c", _NO_SPREAD % {"root": ROOT, "cases": _SPREAD_CASES,
+                                                           "path": path}],
+                       env=env, capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout + r.stderr
+    ref = np.load(path)
+    for i, (dtype, n, batch) in enumerate(_SPREAD_CASES):
+        dt = np.dtype(dtype).type
+        assert T.plan(0 if dt == np.float32 else 1, -1, 2, n, batch)["kind"] == "cooperative"
+        announce("coop spread on", dtype, n, batch)
+        arrs = make_case(n, batch, 2, dt, seed=i + 11)
+        x = run_solve(arrs, n, batch, 2, -1, dt)
+        assert np.array_equal(x, ref["x%d" % i]), (dtype, n, batch)
+        st, _ = check_against_oracle(arrs, x, n, batch, 2, dt)
+        assert st["relerr"] <= TOL[dt], st

@@ -431,6 +431,13 @@ static int32_t coop_pipelined(bool entry_default) {
     return v < 0 ? (entry_default ? 1 : 0) : (v != 0 ? 1 : 0);
 }
 
+// Interleaved groups spread over the batch within a round (coop_group): the W
+// groups a round runs, or 0 with $TRIDIAG_COOP_SPREAD=0 (neighbouring groups).
+static int32_t coop_spread(int groups_per_round) {
+    static const int64_t v = env_int("TRIDIAG_COOP_SPREAD", 1);
+    return v != 0 ? groups_per_round : 0;
+}
+
 // The entry of the cooperative table for (layout, n): interleaved blocks must
 // divide n; K = blocks per system in [2, kmax].  Contiguous systems take it up to
 // 32 blocks (FP32 131072 rows, FP64 65536): measured on MI355X at 2^24 unknowns
@@ -567,7 +574,7 @@ static int coop_run(const SolvePlan<T> &pl, int layout, int64_t n, int64_t batch
     CoopParams<T> p{a, b, c, d, n, batch, il ? 1 : sstride, il ? batch : 1,
                     (int32_t)(il ? n * batch * (int64_t)sizeof(T) : 0), (int32_t)pl.K,
                     (int32_t)ngroups, coop_spin_limit(), coop_pipelined(e.pipe),
-                    reinterpret_cast<T *>(scratch),
+                    il ? coop_spread(pl.grid / pl.K) : 0, reinterpret_cast<T *>(scratch),
                     reinterpret_cast<uint64_t *>(reinterpret_cast<char *>(scratch) +
                                                  pl.forms_bytes),
                     coop_next_tag()};

@@ -19,9 +19,11 @@
 //
 // Schedule: the grid is P workgroups, all resident at once (P is at most what the
 // device holds, and a multiple of K), and workgroup w takes tasks w, w + P,
-// w + 2P, ...  The K tasks of a group are consecutive, so they fall in the same
+// w + 2P, ...  The K tasks of a slot are consecutive, so they fall in the same
 // round and every member publishes before it waits: with all P workgroups
-// resident, every wait ends.  Each workgroup keeps the next round's block in
+// resident, every wait ends.  Slot s solves group s of a contiguous batch; in an
+// interleaved batch the slots of a round take groups spread over the whole batch
+// (coop_group), so that a round reads whole interleaved rows rather than a strip.  Each workgroup keeps the next round's block in
 // flight (in registers) while it waits and finishes the current task, so the
 // memory system stays busy while groups wait for their slowest member.  Every
 // wait is still bounded (spin_limit polls): a workgroup whose partners do not
@@ -48,6 +50,7 @@
 // warp (Tridiagonal_CUDA/tridiagonal_wm_pcr.cu:17-205) or reduce in global
 // memory level by level (tridiagonal_basic.cu:32-164).
 #pragma once
+#include "tri_coop_sched.hpp"
 #include "tri_tile.hpp"
 
 namespace tri {
@@ -63,6 +66,7 @@ template <typename T> struct CoopParams {
     int32_t ngroups;     // groups of S systems
     int32_t spin_limit;  // polls of a wait before it gives up
     int32_t pipelined;   // 1: coop_pipe_kernel (wait one task later), 0: coop_kernel
+    int32_t spread;      // groups per round whose slots are spread over the batch (<= 1: none)
     T *forms;            // [ngroups][K][S][6] block outer forms
     uint64_t *flags;     // [kCoopFlagStride * g + j]: the tag once block j of group g is published
     uint64_t tag;        // this solve's tag: different from every earlier solve's
@@ -79,6 +83,14 @@ constexpr int kCoopFlagStride = 32;
 #ifndef TRI_COOP_POLL_SLEEP
 #define TRI_COOP_POLL_SLEEP 8
 #endif
+
+// Group solved by slot s: interleaved groups are spread over the batch within a
+// round (coop_group); contiguous groups hold one system each and keep slot order.
+template <bool IL, typename T>
+__device__ __forceinline__ int coop_gi(const CoopParams<T> &p, int s) {
+    if constexpr (IL) return coop_group(s, p.ngroups, p.spread);
+    else return s;
+}
 
 template <typename T, int M, int NW, int S, bool IL> struct CoopShape {
     static constexpr int SW = IL ? 64 / S : 64;      // segments of a system per wave
@@ -280,7 +292,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(OC
     int task = (int)blockIdx.x;
     u32x4 g[4][NQ];
     auto issue = [&](int t) {
-        const int g0 = t / K, j0 = t - (t / K) * K;
+        const int s0 = t / K, j0 = t - s0 * K;
+        const int g0 = coop_gi<IL>(p, s0);
         __amdgpu_buffer_rsrc_t rr[4];
         coop_rsrc<T, IL>(p, g0, (int64_t)j0 * R, rr);
         if constexpr (IL) il_block_issue<T, M, SW, S, true>(rr, (int64_t)j0 * R + wrow, p.rstride,
@@ -290,7 +303,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(OC
     if (PF && task < ntask) issue(task);
     for (; task < ntask; task += P) {
         if constexpr (!PF) issue(task);
-        const int gi = task / K, j = task - gi * K;
+        const int si = task / K, j = task - si * K;
+        const int gi = coop_gi<IL>(p, si);
         const int64_t brow = (int64_t)j * R;               // first row of the block
         const int64_t row0 = brow + wrow + (int64_t)kw * M;   // first row of the lane
         const int64_t len = IL ? R : ((n - brow < R) ? n - brow : R);
@@ -480,7 +494,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)
     int task = (int)blockIdx.x;
     u32x4 g[4][NQ];
     if (task < ntask) {
-        const int g0 = task / K, j0 = task - (task / K) * K;
+        const int s0 = task / K, j0 = task - s0 * K;
+        const int g0 = coop_gi<IL>(p, s0);
         __amdgpu_buffer_rsrc_t rr[4];
         coop_rsrc<T, IL>(p, g0, (int64_t)j0 * R, rr);
         if constexpr (IL) il_block_issue<T, M, SW, S, true>(rr, (int64_t)j0 * R + wrow, p.rstride,
@@ -492,7 +507,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)
     Outer<T> poL{}, poR{};
     T pde[NI], pga[NI], pal[NI];
     for (; task < ntask; task += P) {
-        const int gi = task / K, j = task - gi * K;
+        const int si = task / K, j = task - si * K;
+        const int gi = coop_gi<IL>(p, si);
         const int64_t brow = (int64_t)j * R;
         const int64_t row0 = brow + wrow + (int64_t)kw * M;
         const int64_t len = IL ? R : ((n - brow < R) ? n - brow : R);
@@ -540,7 +556,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)
                                __HIP_MEMORY_SCOPE_AGENT);
         const int next = task + P;
         if (next < ntask) {
-            const int g1 = next / K, j1 = next - (next / K) * K;
+            const int s1 = next / K, j1 = next - s1 * K;
+            const int g1 = coop_gi<IL>(p, s1);
             __amdgpu_buffer_rsrc_t rr[4];
             coop_rsrc<T, IL>(p, g1, (int64_t)j1 * R, rr);
             if constexpr (IL) il_block_issue<T, M, SW, S, true>(rr, (int64_t)j1 * R + wrow,
@@ -548,8 +565,9 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)
             else contig_block_issue<T, M, true>(rr, wrow, lane, g);
         }
         if (pend >= 0) {
-            const int pg = pend / K;
-            coop_finish<T, M, NW, S, IL, KMAX, NI>(p, pg, pend - pg * K, sh_blk[par ^ 1], sh_uv,
+            const int ps = pend / K;
+            coop_finish<T, M, NW, S, IL, KMAX, NI>(p, coop_gi<IL>(p, ps),
+                                                   pend - ps * K, sh_blk[par ^ 1], sh_uv,
                                                    sh_k, &s_ok, xbuf[wave], tid, lane, sl, kw, ws,
                                                    wrow, poL, poR, pde, pga, pal);
         }
@@ -565,8 +583,9 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)
         }
     }
     if (pend >= 0) {
-        const int pg = pend / K;
-        coop_finish<T, M, NW, S, IL, KMAX, NI>(p, pg, pend - pg * K, sh_blk[par ^ 1], sh_uv, sh_k,
+        const int ps = pend / K;
+        coop_finish<T, M, NW, S, IL, KMAX, NI>(p, coop_gi<IL>(p, ps), pend - ps * K,
+                                               sh_blk[par ^ 1], sh_uv, sh_k,
                                                &s_ok, xbuf[wave], tid, lane, sl, kw, ws, wrow, poL,
                                                poR, pde, pga, pal);
     }
