@@ -193,7 +193,7 @@ int tridiag_solve_noalloc(int precision, int algo, int layout, int64_t n_eqt, in
  *                         "R precision n_max shape iface"); a shape or iface
  *                         >= 0 forces it for every system (iface: an interface
  *                         solver, or TRIDIAG_IFACE_PCR_DEPTH(d) for PCR +
- *                         Thomas of depth d <= log2(64 * waves per system)).
+ *                         Thomas of depth d <= log2(lanes per system)).
  *   tridiag_ragged_solve  launches the solve for a planned work buffer (one
  *                         kernel, no host synchronisation, no allocation:
  *                         capturable into a graph and replayable).  Concurrent
@@ -205,6 +205,9 @@ int tridiag_solve_noalloc(int precision, int algo, int layout, int64_t n_eqt, in
  *
  * No counterpart in the reference, whose entry points take one n_eqt for the
  * whole batch (Tridiagonal_CUDA/tridiagonal.h:8-57). */
+/* Shape `index` of the persistent kernel: M rows per lane, NWS waves per system
+ * (0 for the sub-wave shapes, which hold 512/S lanes per system and several
+ * systems per wave), S systems per 512-lane workgroup, capacity lanes * M rows. */
 int64_t tridiag_ragged_work_size(int64_t n_sys);
 int tridiag_ragged_shape_count(void);
 int tridiag_ragged_shape_info(int index, int *M, int *NWS, int *S, int64_t *cap);

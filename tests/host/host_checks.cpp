@@ -38,11 +38,15 @@ static void check_schedule(const std::vector<int64_t> &off, int prec, int fs, in
     CHECK(h.magic == kRaggedMagic && h.n_sys == n_sys && h.precision == prec, "header fields");
     CHECK(h.counter == 0 && h.done == 0, "counters not zero");
     CHECK(h.rows == (n_sys > 0 ? off[n_sys] : 0), "rows");
-    CHECK((int64_t)img.size() == kRaggedHeaderBytes + (int64_t)kRaggedRecInts * 4 * h.nitems,
-          "image size %zu for %lld items", img.size(), (long long)h.nitems);
+    CHECK((int64_t)img.size() == kRaggedHeaderBytes + (int64_t)kRaggedRecInts * 4 * h.nitems +
+                                     (int64_t)kRaggedTabInts * 4 * h.ntab,
+          "image size %zu for %lld items, %lld table entries", img.size(), (long long)h.nitems,
+          (long long)h.ntab);
     CHECK((int64_t)img.size() <= ragged_work_bytes(n_sys), "image larger than the work buffer");
     std::vector<int> seen((size_t)n_sys, 0);
     const int32_t *recs = reinterpret_cast<const int32_t *>(img.data() + kRaggedHeaderBytes);
+    const int32_t *tab = recs + (size_t)kRaggedRecInts * h.nitems;
+    int64_t tab_next = 0;
     int last_cap = 1 << 30;
     for (int64_t it = 0; it < h.nitems; ++it) {
         const int32_t *r = recs + (size_t)kRaggedRecInts * it;
@@ -55,9 +59,20 @@ static void check_schedule(const std::vector<int64_t> &off, int prec, int fs, in
         last_cap = ragged_cap(shape);
         if (fs >= 0) CHECK(shape == fs, "forced shape ignored");
         if (fi >= 0) CHECK(iface == fi, "forced iface ignored");
+        const bool sub = kRaggedShapeTab[shape].NWS == 0;
+        if (sub) {
+            // sub-wave items: consecutive, non-overlapping runs of the system table
+            CHECK(r[3] == tab_next && tab_next + count <= h.ntab, "table run %d of %d", r[3], count);
+            if (r[3] != tab_next || tab_next + count > h.ntab) return;
+            tab_next += count;
+        } else {
+            CHECK(r[3] == 0 && count <= kRaggedMaxSlots, "record slots");
+        }
         for (int q = 0; q < count; ++q) {
-            const int64_t o = (int64_t)(((uint64_t)(uint32_t)r[5 + 3 * q] << 32) | (uint32_t)r[4 + 3 * q]);
-            const int64_t n = r[6 + 3 * q];
+            const int32_t *e = sub ? tab + (size_t)kRaggedTabInts * (r[3] + q) : r + 4 + 3 * q;
+            const int64_t o = (int64_t)(((uint64_t)(uint32_t)e[1] << 32) | (uint32_t)e[0]);
+            const int64_t n = e[2];
+            if (sub) CHECK(e[3] == 0, "table padding");
             CHECK(n >= 1 && n <= ragged_cap(shape), "system of %lld rows in capacity %d",
                   (long long)n, ragged_cap(shape));
             // which system starts at o with n rows (offsets are non-decreasing)
@@ -68,6 +83,7 @@ static void check_schedule(const std::vector<int64_t> &off, int prec, int fs, in
             if (s < n_sys) ++seen[(size_t)s];
         }
     }
+    CHECK(tab_next == h.ntab, "table entries not all used");
     for (int64_t s = 0; s < n_sys; ++s) {
         const int64_t n = off[s + 1] - off[s];
         CHECK(seen[(size_t)s] == (n > 0 ? 1 : 0), "system %lld seen %d times", (long long)s,
@@ -104,6 +120,16 @@ static void ragged_cases() {
                            : ragged_schedule(prec, n_sys, off.data(), -1, -1, dflt, img, err);
         CHECK(rc == 0, "schedule failed: %s", err.c_str());
         if (rc == 0) check_schedule(off, prec, -1, -1, img);
+        // every shape forced on the systems it holds
+        for (int k = 0; k < kRaggedShapes; ++k) {
+            std::vector<int64_t> fit;
+            for (auto v : sizes) fit.push_back(std::min<int64_t>(v, ragged_cap(k)));
+            const auto off2 = offsets(fit, start);
+            img.clear();
+            rc = ragged_schedule(prec, n_sys, off2.data(), k, -1, dflt, img, err);
+            CHECK(rc == 0, "forced shape %d failed: %s", k, err.c_str());
+            if (rc == 0) check_schedule(off2, prec, k, -1, img);
+        }
         // forced shape of the largest capacity, forced interface solver
         int big = 0;
         for (int k = 0; k < kRaggedShapes; ++k) if (ragged_cap(k) > ragged_cap(big)) big = k;

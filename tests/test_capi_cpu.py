@@ -89,11 +89,15 @@ def test_planner_covers_benchmark_shapes(capi, prec, layout):
     for n in (1, 2, 3, 31, 64, 100, 512, 1000, 1024, 4096):
         p = capi.plan(prec, 9, layout, n, 4096)
         S, NW, M = p["S"], p["NW"], p["M"]
-        cap = (64 // S) * NW * M if layout == 2 else 64 * (NW // S) * M
+        if p["kind"] == "cooperative":      # with a GPU: K blocks of block_rows rows
+            cap = p["block_rows"] * p["K"]
+        else:
+            cap = (64 // S) * NW * M if layout == 2 else 64 * (NW // S) * M
         assert cap >= n
-    if layout == 2:
-        # 32-byte row groups per workgroup in the interleaved layout
-        assert capi.plan(prec, 9, 2, 4096, 4096)["S"] * (4 if prec == 0 else 8) == 32
+    p = capi.plan(prec, 9, 2, 4096, 4096)
+    if layout == 2 and p["kind"] == "tile":
+        # 32-byte row groups per workgroup in the interleaved tiles
+        assert p["S"] * (4 if prec == 0 else 8) == 32
 
 
 def test_register_packed_cr_rows_per_lane(capi):
@@ -108,7 +112,7 @@ def test_ragged_buckets_hold_their_systems(capi, prec):
         shape, iface = capi.ragged_bucket(prec, n)
         assert shapes[shape]["cap"] >= n
         assert 0 <= iface <= 4
-        assert 64 * shapes[shape]["NWS"] * shapes[shape]["S"] == 512
+        assert shapes[shape]["lanes"] * shapes[shape]["S"] == 512
     with pytest.raises(capi.TridiagError):
         capi.ragged_bucket(prec, 8193)
 

@@ -140,13 +140,13 @@ def test_forced_shape_and_interface(prec, iface):
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_pcr_depths(prec):
-    """PCR + Thomas at PCR depths 1 .. log2(64 NWS) on every shape (interface-solver
+    """PCR + Thomas at PCR depths 1 .. log2(lanes per system) on every shape (interface-solver
     code 3 | depth << 4), and a depth past the shape's segments refused."""
     import torch
     import tridiagonalsolver_amd as T
     dtype = DT[prec]
     for sh in _shapes():
-        cap, top = sh["cap"], (64 * sh["NWS"]).bit_length() - 1
+        cap, top = sh["cap"], sh["lanes"].bit_length() - 1
         rng = np.random.default_rng(7 * sh["index"])
         sizes = np.concatenate([[cap, 1, cap - 1], rng.integers(1, cap + 1, size=2 * sh["S"])])
         off, arrs = _case(sizes, dtype, seed=sh["index"] + 3)
@@ -170,7 +170,7 @@ def test_bit_identical_to_plain_tile(prec):
     checked = 0
     for sh in _shapes():
         cands = [t for t in tiles if not t["interleaved"] and t["padded"] and not t["wide_addr"]
-                 and t["M"] == sh["M"] and t["NW"] // t["S"] == sh["NWS"]
+                 and sh["NWS"] > 0 and t["M"] == sh["M"] and t["NW"] // t["S"] == sh["NWS"]
                  and not t["vec"]]
         if not cands:
             continue

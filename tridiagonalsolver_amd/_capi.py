@@ -175,8 +175,9 @@ def tiles(precision):
 
 
 def ragged_shapes():
-    """Shapes of the ragged persistent kernel: rows per lane M, waves per system NWS,
-    systems per workgroup S and capacity 64*NWS*M."""
+    """Shapes of the ragged persistent kernel: rows per lane M, waves per system NWS
+    (0: a sub-wave shape of 512/S lanes per system), systems per workgroup S, lanes
+    (interface segments) per system and capacity lanes*M."""
     L = lib()
     out = []
     for i in range(L.tridiag_ragged_shape_count()):
@@ -184,7 +185,8 @@ def ragged_shapes():
         cap = ctypes.c_int64()
         check(L.tridiag_ragged_shape_info(i, ctypes.byref(M), ctypes.byref(NWS), ctypes.byref(S),
                                           ctypes.byref(cap)))
-        out.append(dict(index=i, M=M.value, NWS=NWS.value, S=S.value, cap=cap.value))
+        out.append(dict(index=i, M=M.value, NWS=NWS.value, S=S.value, cap=cap.value,
+                        lanes=cap.value // M.value))
     return out
 
 

@@ -155,11 +155,10 @@ def _ragged_inputs(torch, prec, sizes, seed):
     return off, [a, b, c, d]
 
 
-def iface_codes(nws, depths=None):
-    """Interface-solver codes a ragged shape of `nws` waves per system takes: the five
-    families at their defaults, then PCR + Thomas at PCR depths 1 .. log2(64 nws)
-    (or the given depths, where valid)."""
-    segs = 64 * nws
+def iface_codes(segs, depths=None):
+    """Interface-solver codes a ragged shape of `segs` lanes (segments) per system
+    takes: the five families at their defaults, then PCR + Thomas at PCR depths
+    1 .. log2(segs) (or the given depths, where valid)."""
     top = segs.bit_length() - 1
     ds = range(1, top + 1) if depths is None else [d for d in depths if 1 <= d <= top]
     return list(range(5)) + [3 | (d << 4) for d in ds]
@@ -197,7 +196,7 @@ def ragged_sweep(prec, kmin=5, kmax=13, rows_per_class=1 << 23, reps=10, seed=3,
         for sh in shapes:
             if sh["cap"] < hi:
                 continue
-            for iface in iface_codes(sh["NWS"], pcr_depths):
+            for iface in iface_codes(sh["lanes"], pcr_depths):
                 try:
                     plan = RaggedPlan(off, dtype=dtype, shape=sh["index"], iface=iface)
                 except _capi.TridiagError as e:

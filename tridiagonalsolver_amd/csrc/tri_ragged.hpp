@@ -161,6 +161,109 @@ __device__ __forceinline__ void ragged_item(const RaggedParams<T> &p, const int3
     }
 }
 
+// Rows j0 .. j0+CH-1 of the lane's own system through 64-bit addresses (the
+// systems of one wave differ, so no wave-uniform buffer descriptor covers them):
+// whole 16-byte vectors where they lie inside the system (4-byte aligned, as the
+// systems start anywhere), single elements at its end, identity rows past it.
+template <typename T, int CH> struct LaneRowLoader {
+    const T *a, *b, *c, *d;      // row 0 of the lane's system
+    int64_t row0, n;
+    __device__ __forceinline__ void load(T (&q)[4][CH], int j0) const {
+        constexpr int V = 16 / (int)sizeof(T);
+        static_assert(CH % V == 0, "rows move in 16-byte vectors");
+        typedef unsigned int u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+        const T *src[4] = {a, b, c, d};
+#pragma unroll
+        for (int v0 = 0; v0 < CH; v0 += V) {
+            const int64_t r = row0 + j0 + v0;
+            if (r + V <= n) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const u32x4 t = *reinterpret_cast<const u32x4a4 *>(src[k] + r);
+#pragma unroll
+                    for (int v = 0; v < V; ++v) q[k][v0 + v] = reinterpret_cast<const T *>(&t)[v];
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    if (r + v < n) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) q[k][v0 + v] = src[k][r + v];
+                    } else {
+                        q[0][v0 + v] = 0; q[1][v0 + v] = 1; q[2][v0 + v] = 0; q[3][v0 + v] = 0;
+                    }
+                }
+            }
+        }
+    }
+};
+
+// One item of a sub-wave shape (M rows per lane, S systems of L = 512/S lanes,
+// LW = 64/L systems per wave).  Lane = kw*LW + the system's place in its wave, the
+// lane order of the interleaved tiles, so that the Wang-Mou merges and the LDS
+// interface solvers run as they are with L segments per system and no cross-wave
+// chain.  The item's systems are entries tab[0 .. count) of the system table.
+// Short systems in the wave shapes leave most lanes holding padding rows and pay
+// a workgroup-wide solve per 8 systems; here a 512-lane item holds 32 or 64.
+template <typename T, int M, int S>
+__device__ __forceinline__ void ragged_item_sub(const RaggedParams<T> &p, const int32_t *tab,
+                                                int iface, int count, char *iface_lds) {
+    constexpr int L = kRaggedLanes / S;
+    constexpr int LW = 64 / L;
+    static_assert(L * S == kRaggedLanes && LW * L == 64, "sub-wave shape");
+    using Sh = TileShape<T, M, 8, 8, false, true, false, false, true, true, false>;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63, wave = tid >> 6;
+    const int kw = lane / LW, slw = lane % LW;
+    const int sl = wave * LW + slw;
+    const bool valid = sl < count;
+    int64_t off = 0, n = 0;
+    if (valid) {
+        const int32_t *e = tab + (int64_t)kRaggedTabInts * sl;
+        off = (int64_t)(((uint64_t)(uint32_t)e[1] << 32) | (uint32_t)e[0]);
+        n = e[2];
+    }
+    const int64_t row0 = (int64_t)kw * M;
+    T al_r[Sh::NI];
+    auto al_store = [&](int j, T v) { al_r[j] = v; };
+    SweepOut<T, Sh::NI> o;
+    LaneRowLoader<T, Sh::CH> ld{p.a + off, p.b + off, p.c + off, p.d + off, row0, n};
+    lane_sweep<T, Sh>(o, ld, row0, n, al_store);
+    const T DLr = o.bL - o.cL * o.SA, ELr = o.cL * o.PI, FLr = o.dL - o.cL * o.SD;
+    const T ARr = -(o.aR * o.alp), DRr = o.bR - o.aR * o.g, FRr = o.dR - o.aR * o.del;
+    T xL, xR;
+    interface_solve<T, S, L, 1, LW, L, kRaggedLanes>(
+        o.aL, DLr, ELr, FLr, ARr, DRr, o.cR, FRr, tid, lane, sl, kw, 0, iface, 0, iface_lds, xL, xR);
+    if (!valid || row0 >= n) return;
+    T xs[M];
+    xs[0] = xL;
+    xs[M - 1] = xR;
+    T xn = xR;
+#pragma unroll
+    for (int j = M - 3; j >= 0; --j) {
+        xn = (o.de[j] - al_r[j] * xL) - o.ga[j] * xn;
+        xs[j + 1] = xn;
+    }
+    constexpr int V = 16 / (int)sizeof(T);
+    typedef unsigned int u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+    T *x = p.d + off;
+#pragma unroll
+    for (int v0 = 0; v0 < M; v0 += V) {
+        const int64_t r = row0 + v0;
+        if (r + V <= n) {
+            u32x4 t;
+#pragma unroll
+            for (int v = 0; v < V; ++v) reinterpret_cast<T *>(&t)[v] = xs[v0 + v];
+            *reinterpret_cast<u32x4a4 *>(x + r) = t;
+        } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+                if (r + v < n) x[r + v] = xs[v0 + v];
+        }
+    }
+}
+
 // Dynamic LDS of the persistent kernel: the per-wave transposition blocks, then
 // room for the LDS interface solvers.
 template <typename T> constexpr int ragged_xbuf_bytes() {
@@ -181,6 +284,7 @@ __global__ void __launch_bounds__(kRaggedLanes) ragged_kernel(RaggedParams<T> p)
     if (h->magic != kRaggedMagic || h->n_sys != p.n_sys || h->precision != p.precision) return;
     const int nitems = (int)h->nitems;
     const int64_t rows = h->rows;
+    const int32_t *tab = p.recs + (int64_t)kRaggedRecInts * nitems;   // system table
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     u32x4 *xbuf = reinterpret_cast<u32x4 *>(tri_dyn_lds) +
                   (size_t)wave * (64 * kRaggedMaxM * sizeof(T) / 16);
@@ -218,19 +322,28 @@ __global__ void __launch_bounds__(kRaggedLanes) ragged_kernel(RaggedParams<T> p)
         const int shape = __builtin_amdgcn_readfirstlane(rec[0]);
         const int iface = __builtin_amdgcn_readfirstlane(rec[1]);
         const int count = __builtin_amdgcn_readfirstlane(rec[2]);
+        const int first = __builtin_amdgcn_readfirstlane(rec[3]);
         switch (shape) {
 #define TRI_RAGGED_CASE(K)                                                                   \
     case K:                                                                                \
-        ragged_item<T, kRaggedShapeTab[K].M, kRaggedShapeTab[K].NWS, kRaggedShapeTab[K].S>( \
-            p, rec, iface, count, rows, xbuf, iface_lds);                                  \
+        if constexpr (kRaggedShapeTab[K].NWS == 0)                                         \
+            ragged_item_sub<T, kRaggedShapeTab[K].M, kRaggedShapeTab[K].S>(                \
+                p, tab + (int64_t)kRaggedTabInts * first, iface, count, iface_lds);        \
+        else                                                                               \
+            ragged_item<T, kRaggedShapeTab[K].M, kRaggedShapeTab[K].NWS, kRaggedShapeTab[K].S>( \
+                p, rec, iface, count, rows, xbuf, iface_lds);                              \
         break;
             TRI_RAGGED_CASE(0) TRI_RAGGED_CASE(1) TRI_RAGGED_CASE(2) TRI_RAGGED_CASE(3)
             TRI_RAGGED_CASE(4) TRI_RAGGED_CASE(5) TRI_RAGGED_CASE(6) TRI_RAGGED_CASE(7)
             TRI_RAGGED_CASE(8) TRI_RAGGED_CASE(9) TRI_RAGGED_CASE(10) TRI_RAGGED_CASE(11)
             TRI_RAGGED_CASE(12) TRI_RAGGED_CASE(13) TRI_RAGGED_CASE(14) TRI_RAGGED_CASE(15)
+            TRI_RAGGED_CASE(16) TRI_RAGGED_CASE(17) TRI_RAGGED_CASE(18) TRI_RAGGED_CASE(19)
+            TRI_RAGGED_CASE(20) TRI_RAGGED_CASE(21) TRI_RAGGED_CASE(22) TRI_RAGGED_CASE(23)
+            TRI_RAGGED_CASE(24)
 #undef TRI_RAGGED_CASE
         default: break;
         }
+        static_assert(kRaggedShapes == 25, "one case per shape");
         if (wave == 0) {
             if (lane < kRaggedRecInts) s_rec[buf ^ 1][lane] = rnext;
             if (lane == 0) s_idx[buf ^ 1] = nxt;

@@ -11,19 +11,31 @@
 
 namespace tri {
 
-// Shapes of the ragged kernel (512 lanes): {M, NWS, S}; capacity 64*NWS*M rows.
+// Shapes of the ragged kernel (512 lanes): {M, NWS, S}, S systems per workgroup,
+// M rows per lane.  NWS >= 1: NWS waves (64*NWS lanes) per system.  NWS = 0: a
+// sub-wave shape, 512/S lanes per system and 64*S/512 systems per wave.  Capacity:
+// lanes per system * M rows.
 struct RaggedShape {
     int M, NWS, S;
 };
-constexpr int kRaggedShapes = 16;
+constexpr int kRaggedLanes = 512;
+constexpr int kRaggedShapes = 25;
 constexpr RaggedShape kRaggedShapeTab[kRaggedShapes] = {
     {2, 1, 8},  {4, 1, 8},  {8, 1, 8},  {8, 2, 4},  {16, 1, 8}, {8, 4, 2},
     {16, 2, 4}, {8, 8, 1},  {16, 4, 2}, {16, 8, 1}, {4, 2, 4},  {4, 4, 2},
     // 12 rows per lane: capacities 1.5x a power of two, so that systems between
     // two powers of two leave fewer padding rows
-    {12, 1, 8}, {12, 2, 4}, {12, 4, 2}, {12, 8, 1}};
-constexpr int ragged_cap(int k) { return 64 * kRaggedShapeTab[k].NWS * kRaggedShapeTab[k].M; }
-constexpr int kRaggedLanes = 512;
+    {12, 1, 8}, {12, 2, 4}, {12, 4, 2}, {12, 8, 1},
+    // sub-wave shapes for short systems: 16 lanes (capacities 64, 128, 256) or 8
+    // lanes (32, 64) per system
+    {4, 0, 32}, {8, 0, 32}, {16, 0, 32}, {4, 0, 64}, {8, 0, 64},
+    // 32 lanes (capacities 256, 384, 512) and 16 lanes of 12 rows (192)
+    {8, 0, 16}, {12, 0, 16}, {16, 0, 16}, {12, 0, 32}};
+// Lanes (segments) per system of shape k.
+constexpr int ragged_lanes(int k) {
+    return kRaggedShapeTab[k].NWS ? 64 * kRaggedShapeTab[k].NWS : kRaggedLanes / kRaggedShapeTab[k].S;
+}
+constexpr int ragged_cap(int k) { return ragged_lanes(k) * kRaggedShapeTab[k].M; }
 constexpr int kRaggedWaves = kRaggedLanes / 64;
 constexpr int kRaggedMaxN = 64 * 8 * 16;
 constexpr int kRaggedMaxM = 16;
@@ -42,12 +54,15 @@ inline bool iface_code_valid(int code, int P) {
     return fam == 3 && (1 << (depth - 1)) <= P / 2;
 }
 // Lane segments per system of ragged shape k.
-constexpr int ragged_segments(int k) { return 64 * kRaggedShapeTab[k].NWS; }
+constexpr int ragged_segments(int k) { return ragged_lanes(k); }
 
-// Work buffer: a 64-byte header, then one 128-byte record per item:
-//   r[0] shape, r[1] interface solver, r[2] number of systems, r[3] 0,
+// Work buffer: a 64-byte header, one 128-byte record per item, then the system
+// table of the sub-wave items (16 bytes per system):
+//   r[0] shape, r[1] interface solver, r[2] number of systems,
+//   r[3] (sub-wave shapes) index of the item's first system-table entry, else 0,
 //   r[4 + 3k], r[5 + 3k]  first row of system slot k (int64, low word first),
-//   r[6 + 3k]             rows of system slot k.
+//   r[6 + 3k]             rows of system slot k (shapes with NWS >= 1);
+//   table entry: first row (int64, low word first), rows, 0.
 // Written by the host planner once (with both counters zero).  A solve leaves the
 // header as it found it: the last workgroup to finish zeroes both counters.
 struct RaggedHeader {
@@ -58,15 +73,19 @@ struct RaggedHeader {
     int32_t precision;
     int32_t done;        // workgroups of the running solve that have left the queue
     int64_t rows;        // offsets[n_sys]: rows of the arrays that the systems cover
+    int64_t ntab;        // entries of the system table (after the nitems records)
 };
 constexpr int64_t kRaggedMagic = 0x52414747454432LL;
 constexpr int64_t kRaggedHeaderBytes = 64;
 static_assert(sizeof(RaggedHeader) <= kRaggedHeaderBytes, "header exceeds its 64 bytes");
 constexpr int kRaggedRecInts = 32;
 constexpr int kRaggedMaxSlots = 8;
+constexpr int kRaggedTabInts = 4;
 
+// Upper bound of the work buffer: one record per system (every item holds at
+// least one) and one table entry per system.
 inline int64_t ragged_work_bytes(int64_t n_sys) {
-    const int64_t raw = kRaggedHeaderBytes + (int64_t)kRaggedRecInts * 4 * n_sys;
+    const int64_t raw = kRaggedHeaderBytes + (int64_t)(kRaggedRecInts + kRaggedTabInts) * 4 * n_sys;
     return (raw + 255) & ~(int64_t)255;
 }
 
@@ -155,27 +174,40 @@ int ragged_schedule(int prec, int64_t n_sys, const int64_t *off, int force_shape
     }
     std::vector<char> host((size_t)ragged_work_bytes(n_sys), 0);
     int32_t *recs = reinterpret_cast<int32_t *>(host.data() + kRaggedHeaderBytes);
+    std::vector<int32_t> tab;
     int64_t nitems = 0;
     for (int k : order) {
-        const int S = kRaggedShapeTab[k / NF].S;
+        const int shape = k / NF;
+        const int S = kRaggedShapeTab[shape].S;
+        const bool sub = kRaggedShapeTab[shape].NWS == 0;
         for (int64_t i = 0; i < cnt[(size_t)k]; i += S) {
             int32_t *r = recs + (size_t)kRaggedRecInts * nitems++;
             const int count = (int)std::min<int64_t>(S, cnt[(size_t)k] - i);
-            r[0] = k / NF;
+            r[0] = shape;
             r[1] = k % NF;
             r[2] = count;
+            r[3] = sub ? (int32_t)(tab.size() / kRaggedTabInts) : 0;
             for (int q = 0; q < count; ++q) {
                 const int64_t s = perm[(size_t)(first[(size_t)k] + i + q)];
                 const uint64_t o = (uint64_t)off[s];
-                r[4 + 3 * q] = (int32_t)(uint32_t)(o & 0xffffffffu);
-                r[5 + 3 * q] = (int32_t)(uint32_t)(o >> 32);
-                r[6 + 3 * q] = (int32_t)(off[s + 1] - off[s]);
+                const int32_t lo = (int32_t)(uint32_t)(o & 0xffffffffu), hi = (int32_t)(uint32_t)(o >> 32);
+                const int32_t rows = (int32_t)(off[s + 1] - off[s]);
+                if (sub) {
+                    tab.insert(tab.end(), {lo, hi, rows, 0});
+                } else {
+                    r[4 + 3 * q] = lo;
+                    r[5 + 3 * q] = hi;
+                    r[6 + 3 * q] = rows;
+                }
             }
         }
     }
-    const RaggedHeader h{kRaggedMagic, n_sys, nitems, 0, prec, 0, n_sys > 0 ? off[n_sys] : 0};
+    const int64_t ntab = (int64_t)tab.size() / kRaggedTabInts;
+    const RaggedHeader h{kRaggedMagic, n_sys, nitems, 0, prec, 0, n_sys > 0 ? off[n_sys] : 0, ntab};
     memcpy(host.data(), &h, sizeof h);
-    host.resize((size_t)(kRaggedHeaderBytes + (int64_t)kRaggedRecInts * 4 * nitems));
+    const int64_t rec_bytes = kRaggedHeaderBytes + (int64_t)kRaggedRecInts * 4 * nitems;
+    if (ntab) memcpy(host.data() + rec_bytes, tab.data(), tab.size() * sizeof(int32_t));
+    host.resize((size_t)(rec_bytes + ntab * kRaggedTabInts * 4));
     image.swap(host);
     return 0;
 }
