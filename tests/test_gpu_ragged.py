@@ -331,3 +331,30 @@ def test_misaligned_arrays(prec):
     torch.cuda.synchronize()
     assert float(big[3][0]) == 0.0                       # nothing written before the array
     _check(off, arrs, views[3].cpu().numpy(), dtype)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_sub_wave_slots_give_identical_bits(prec):
+    """The sub-wave shapes hold 16-64 systems per item, several per wave, read through
+    per-lane addresses.  One system copied into every slot of several items (and at
+    every 4-byte misalignment of its first row) must solve to the same bits in every
+    copy, for each sub-wave shape and interface solver, and match the oracle."""
+    dtype = DT[prec]
+    for sh in _shapes():
+        if sh["NWS"] != 0:
+            continue
+        for n in sorted({sh["cap"], sh["cap"] - 1, max(2, sh["cap"] // 2 + 3)}):
+            a, b, c, d = oracle.gen_dd(n, 1, layout=0, seed=n + sh["index"])
+            reps = 3 * sh["S"] + 5
+            lead = [1, 2, 3]                       # shift the copies off 16-byte boundaries
+            sizes = np.array(lead + [n] * reps, dtype=np.int64)
+            off = oracle.offsets_from_sizes(sizes)
+            head = [oracle.gen_dd(m, 1, layout=0, seed=m) for m in lead]
+            arrs = tuple(np.concatenate([h[k] for h in head] + [v] * reps).astype(dtype)
+                         for k, v in enumerate((a, b, c, d)))
+            for iface in (0, 1, 3):
+                announce("ragged sub-wave replicas", prec, sh, n, iface)
+                x = _run(off, arrs, shape=sh["index"], iface=iface)
+                copies = x[sum(lead):].reshape(reps, n)
+                assert (copies == copies[:1]).all(), (sh, n, iface)
+                _check(off, arrs, x, dtype)
