@@ -1,9 +1,10 @@
 """GPU parity of the cooperative row-block solve (K workgroups per group of
 systems, block forms exchanged through global memory), through the C-ABI,
 against the FP64 oracle: normwise relative error <= 1e-5 (FP32) / 1e-12 (FP64)
-over every system.  Interleaved systems take it from 2048 rows when 256-row
-blocks divide n, contiguous systems above one
-workgroup's rows in up to 32 blocks; every case asserts the plan it tests.  A
+over every system.  Interleaved systems take it from 1024 rows when 256-row
+blocks divide n (and, up to the 4096 rows the interleaved tiles hold, when rows
+start on 128-byte lines), contiguous systems above one workgroup's rows in up
+to 32 blocks; every case asserts the plan it tests.  A
 forced zero spin limit (run in a child process) checks that a wait that gives up
 ends the kernel and marks its block with NaN, never with a wrong finite value."""
 import os
@@ -37,10 +38,10 @@ def _assert_coop(prec, layout, n, batch, K):
 
 
 @pytest.mark.parametrize("dtype,n,batch,K", [
-    (np.float32, 2048, 33, 8), (np.float32, 2560, 70, 10), (np.float32, 4096, 64, 16),
-    (np.float32, 4096, 4096, 16), (np.float32, 6144, 31, 24), (np.float32, 8192, 40, 32),
-    (np.float64, 4096, 17, 16), (np.float64, 4096, 4096, 16), (np.float64, 5120, 40, 20),
-    (np.float64, 8192, 33, 32)])
+    (np.float32, 1024, 64, 4), (np.float32, 2048, 96, 8), (np.float32, 2560, 160, 10),
+    (np.float32, 4096, 64, 16), (np.float32, 4096, 4096, 16), (np.float32, 6144, 31, 24),
+    (np.float32, 8192, 40, 32), (np.float64, 1280, 48, 5), (np.float64, 4096, 48, 16),
+    (np.float64, 4096, 4096, 16), (np.float64, 5120, 40, 20), (np.float64, 8192, 33, 32)])
 def test_interleaved(dtype, n, batch, K):
     _assert_coop(0 if dtype == np.float32 else 1, 2, n, batch, K)
     announce("coop interleaved", np.dtype(dtype).name, n, batch)
@@ -78,9 +79,15 @@ def test_contiguous(dtype, n, batch, K):
 def test_plan_boundaries():
     """Where the planner switches to and from the cooperative solve."""
     import tridiagonalsolver_amd as T
-    assert T.plan(0, -1, 2, 1024, 64)["kind"] == "tile"
-    assert T.plan(1, -1, 2, 1024, 64)["kind"] == "tile"
+    assert T.plan(0, -1, 2, 768, 64)["kind"] == "tile"
+    assert T.plan(1, -1, 2, 768, 64)["kind"] == "tile"
+    assert T.plan(0, -1, 2, 1024, 64)["kind"] == "cooperative"
+    assert T.plan(1, -1, 2, 1024, 64)["kind"] == "cooperative"
     assert T.plan(1, -1, 2, 2048, 64)["kind"] == "cooperative"
+    # rows off 128-byte lines keep the tile where one holds n, not beyond
+    assert T.plan(0, -1, 2, 4096, 4001)["kind"] == "tile"
+    assert T.plan(1, -1, 2, 2048, 17)["kind"] == "tile"
+    assert T.plan(0, -1, 2, 8192, 4001)["kind"] == "cooperative"
     assert T.plan(0, -1, 2, 2304, 64)["kind"] == "cooperative"
     assert T.plan(0, -1, 2, 2100, 64)["kind"] == "tile"             # blocks must divide n
     assert T.plan(0, -1, 2, 8448, 64)["kind"] == "via_contiguous"   # 33 blocks
@@ -191,7 +198,7 @@ for i, (dtype, n, batch) in enumerate(%(cases)r):
 np.savez(%(path)r, **out)
 print("ok", flush=True)
 """
-_SPREAD_CASES = [("float32", 4096, 4096), ("float64", 4096, 1000), ("float32", 2048, 3000),
+_SPREAD_CASES = [("float32", 4096, 4096), ("float64", 4096, 1008), ("float32", 2048, 3008),
                  ("float64", 2048, 16)]
 
 

@@ -39,7 +39,7 @@ def _run(capi, fn, prec, algo, layout, n, batch, arrs):
     (0, 0, -1, [(40000, 24), (16400, 3)]),    # FP32 contiguous
     (0, 2, -1, [(4096, 640), (2048, 40)]),    # FP32 interleaved, cooperative blocks
     (1, 2, -1, [(4096, 320), (1024, 9)]),     # FP64 interleaved
-    (0, 2, -1, [(4096, 2048), (2048, 3000)]),  # groups of a round spread over the batch
+    (0, 2, -1, [(4096, 2048), (2048, 3008)]),  # groups of a round spread over the batch
     (1, 0, 5, [(24581, 40), (9000, 5)]),      # PCR interface: the chunked solve
     (1, 0, -1, [(70000, 6), (66000, 3)]),     # FP64 beyond 32 blocks: the chunked solve
 ])

@@ -388,17 +388,28 @@ static int64_t env_int(const char *name, int64_t dflt) {
 
 // Interleaved systems of at least this many rows take the cooperative solve
 // ($TRIDIAG_COOP_MIN_ROWS overrides for both precisions; a negative value turns
-// the cooperative solve off for both layouts).  Measured on MI355X at 2^24
-// unknowns against the one-workgroup tiles: FP32 1024 rows 0.092 vs 0.091 ms,
-// 1536 rows 0.164 vs 0.146 ms, 2048 rows 0.091 vs 0.122 ms, 4096 rows 0.107 vs
-// 0.118 ms; FP64 2048 rows 0.204 vs 0.213 ms, 3072 rows 0.327 vs 0.358 ms, 4096
-// rows 0.214-0.221 vs 0.253 ms.  Above 4096 rows (no interleaved tile) it
-// replaces the solve through a contiguous copy: 8192 rows 0.123 vs 0.201 ms
-// (FP32), 0.257 vs 0.406 ms (FP64).
+// the cooperative solve off for both layouts).  Measured on MI355X (groups of a
+// round spread over the batch) against the one-workgroup tiles, at about 2^24
+// unknowns: FP32 768 rows 0.082 vs 0.093 ms, 1024 rows 0.082 vs 0.090 ms, 1280
+// rows 0.082 vs 0.143 ms, 1536 rows 0.083 vs 0.117 ms; FP64 1024 rows 0.181 vs
+// 0.218 ms, 1536 rows 0.166 vs 0.251 ms; at 512 rows the tile wins (FP32 0.084 vs
+// 0.078 ms).  At 4M unknowns FP32 1024 rows is a draw (0.028 vs 0.025 ms), FP64
+// 0.045 vs 0.054 ms.  Above 4096 rows (no interleaved tile) it replaces the solve
+// through a contiguous copy: 8192 rows 0.123 vs 0.201 ms (FP32), 0.257 vs 0.406 ms
+// (FP64).
 static int64_t coop_min_rows_il(int prec) {
     static const int64_t env = env_int("TRIDIAG_COOP_MIN_ROWS", 0);
     if (env != 0) return env;
-    return 2048;
+    return 1024;
+}
+
+// Interleaved rows whose pitch is not a multiple of 128 bytes: the groups' row
+// pieces straddle cache lines, which costs the cooperative solve more than the
+// tiles (FP32 4096 x 4001 0.168 vs 0.117 ms, 2048 x 8191 0.157 vs 0.121 ms; FP64
+// 0.349 vs 0.230, 0.316 vs 0.227 ms), so such batches keep the tile wherever one
+// holds n.
+template <typename T> static bool il_rows_on_lines(int64_t batch) {
+    return (batch * (int64_t)sizeof(T)) % 128 == 0;
 }
 // Polls of a cooperative wait before it gives up ($TRIDIAG_COOP_SPIN).
 static int32_t coop_spin_limit() {
@@ -661,7 +672,10 @@ static SolvePlan<T> make_plan(int algo, int layout, int64_t n, int64_t batch, in
         return pl;
     }
     // rows beyond one workgroup: K workgroups per group of systems, on chip
-    if ((il || n > large_min_rows<T>()) && coop_plan<T>(algo, layout, n, batch, fixed_bytes, pl))
+    const bool il_tile_first = il && !il_rows_on_lines<T>(batch) &&
+                               choose_tile<T>(algo, layout, n, vec_ok, bb == 0) != nullptr;
+    if ((il || n > large_min_rows<T>()) && !il_tile_first &&
+        coop_plan<T>(algo, layout, n, batch, fixed_bytes, pl))
         return pl;
     if (!il && n > large_min_rows<T>() && n > large_chunk_rows(prec, 0)) {
         pl.kind = kPlanChunked;
