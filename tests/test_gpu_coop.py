@@ -199,18 +199,19 @@ np.savez(%(path)r, **out)
 print("ok", flush=True)
 """
 _SPREAD_CASES = [("float32", 4096, 4096), ("float64", 4096, 1008), ("float32", 2048, 3008),
-                 ("float64", 2048, 16)]
+                 ("float64", 2048, 16), ("float32", 256, 65536), ("float64", 512, 8192)]
 
 
 def test_group_spread_same_bits(tmp_path):
-    """The schedule spreads the groups of one round over the whole batch
-    (TRIDIAG_COOP_SPREAD, default on).  Each group's arithmetic is unchanged, so
-    x must be bit-identical to the run with neighbouring groups (child process:
-    the setting is read once).  Cases with one partial table column and with a
-    single round."""
+    """The cooperative schedule spreads the groups of one round over the whole batch
+    (TRIDIAG_COOP_SPREAD), and the interleaved tiles of whole 128-byte pieces their
+    resident tiles (TRIDIAG_TILE_SPREAD); both default on.  Each group's and tile's
+    arithmetic is unchanged, so x must be bit-identical to the run in dispatch
+    order (child process: the settings are read once).  Cases with one partial
+    table column, with a single round, and two tile cases."""
     import tridiagonalsolver_amd as T
     path = str(tmp_path / "nospread.npz")
-    env = dict(os.environ, TRIDIAG_COOP_SPREAD="0")
+    env = dict(os.environ, TRIDIAG_COOP_SPREAD="0", TRIDIAG_TILE_SPREAD="0")
     announce("coop spread off (child process)")
     r = subprocess.run([sys.executable, "-c", _NO_SPREAD % {"root": ROOT, "cases": _SPREAD_CASES,
                                                            "path": path}],
@@ -219,7 +220,8 @@ def test_group_spread_same_bits(tmp_path):
     ref = np.load(path)
     for i, (dtype, n, batch) in enumerate(_SPREAD_CASES):
         dt = np.dtype(dtype).type
-        assert T.plan(0 if dt == np.float32 else 1, -1, 2, n, batch)["kind"] == "cooperative"
+        kind = "cooperative" if n >= 1024 else "tile"
+        assert T.plan(0 if dt == np.float32 else 1, -1, 2, n, batch)["kind"] == kind
         announce("coop spread on", dtype, n, batch)
         arrs = make_case(n, batch, 2, dt, seed=i + 11)
         x = run_solve(arrs, n, batch, 2, -1, dt)

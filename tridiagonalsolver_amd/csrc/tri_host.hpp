@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include <string>
 
@@ -47,7 +48,33 @@ hipError_t launch_tile(const TileParams<T> &p, hipStream_t st) {
             granted = dyn;
         }
     }
-    hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(64 * NW), dyn, st, p);
+    TileParams<T> q = p;
+    q.spread = 0;
+    if constexpr (IL && S * (int)sizeof(T) >= 128) {
+        // tiles of whole 128-byte row pieces, resident at once, spread over the batch
+        // when rows start on lines and systems have at least 128 rows.  Measured on
+        // MI355X (ms, in order / spread): FP32 256 x 65536 0.068 / 0.064, 512 x 32768
+        // 0.078 / 0.072; FP64 0.136 / 0.129, 0.158 / 0.148; FP32 64 x 262144 0.082 /
+        // 0.083.  Tiles of narrower pieces share lines with their neighbours
+        // (xcd_group) and lost with it: FP64 2048 x 8191 0.227 / 0.269 ms.
+        // ($TRIDIAG_TILE_SPREAD=0 turns it off.)
+        constexpr int G = 1;
+        static const int resident = [&] {
+            int per_cu = 0, dev = 0, cus = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * NW, dyn) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+                return 0;
+            return per_cu * cus;
+        }();
+        static const bool on = [] {
+            const char *e = getenv("TRIDIAG_TILE_SPREAD");
+            return !(e && atoi(e) == 0);
+        }();
+        if (on && p.n >= 128 && (p.rstride * (int64_t)sizeof(T)) % 128 == 0 && resident / G > 1)
+            q.spread = resident / G;
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(64 * NW), dyn, st, q);
     return hipGetLastError();
 }
 

@@ -34,6 +34,7 @@
 //   !IL: contiguous per system (layout 0, or layout 1 with system stride 4n);
 //        a system is owned by NWS consecutive waves, lane = segment.
 #pragma once
+#include "tri_coop_sched.hpp"
 #include "tri_device.hpp"
 #include "tri_iface.hpp"
 
@@ -63,6 +64,9 @@ template <typename T> struct TileParams {
     int32_t buf_bytes;
     int32_t iface;     // interface solver (Iface)
     int32_t xchg_lds;  // Wang-Mou exchanges through LDS (1) or ds_bpermute (0)
+    // Interleaved tiles: line groups resident at once, whose tiles are spread over the
+    // batch (coop_group); 0: tiles in dispatch order.  Set by launch_tile.
+    int32_t spread = 0;
 };
 
 template <typename T, int M, int NW, int S, bool IL, bool VEC, bool ALDS, bool BIG, bool PAD,
@@ -628,7 +632,16 @@ __global__ void __launch_bounds__(64 * NW)
     const int ws = IL ? wave : (wave % NWS);         // wave-block inside the system
     const int64_t n = prm.n;
     int tile = blockIdx.x;
-    if (IL) tile = xcd_group(blockIdx.x, gridDim.x, 128 / (S * (int)sizeof(T)));
+    if (IL) {
+        // tiles that share 128-byte lines on one XCD, then those line groups spread
+        // over the batch so that the resident ones read whole rows, not a strip
+        constexpr int G = (S * (int)sizeof(T) >= 128) ? 1 : 128 / (S * (int)sizeof(T));
+        tile = xcd_group(blockIdx.x, gridDim.x, G);
+        if (prm.spread > 1) {
+            const int lg = tile / G;
+            tile = coop_group(lg, (int)gridDim.x / G, prm.spread) * G + (tile - lg * G);
+        }
+    }
     const int64_t sys = (int64_t)tile * S + sl;
     const bool valid = sys < prm.batch;
     const int64_t sysc = valid ? sys : prm.batch - 1;   // loads of idle lanes stay in bounds
