@@ -112,6 +112,45 @@ __device__ __forceinline__ int coop_wait(const CoopParams<T> &p, int g, int lane
     }
 }
 
+// Stages the K block forms of a group (K*S*6 values at fg, block-major) into
+// sh_k[system][block][6] with the whole workgroup.  In FP32 each thread issues 8
+// loads before it stores the first: the loads are agent-scope (they go past the
+// XCD's L2 to where all XCDs meet), and one after another their latencies add up.
+// The FP64 kernels have no registers to spare (measured: FP64 65,536 x 256 0.173
+// -> 0.182 ms with the batched form) and load one value at a time.  ok == false
+// (a wait that gave up): zeros, and no loads.
+template <typename T, int S, int NW, int KMAX>
+__device__ __forceinline__ void coop_stage_forms(const T *fg, int K, bool ok, int tid,
+                                                 T (*sh_k)[KMAX][6]) {
+    constexpr int NT = 64 * NW, B = 8;
+    const int ne = K * S * 6;
+    if constexpr (sizeof(T) != 4) {
+        for (int e = tid; e < ne; e += NT) {
+            const int k = e / (S * 6), r = e - k * (S * 6);
+            sh_k[r / 6][k][r % 6] =
+                ok ? __hip_atomic_load(fg + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : T(0);
+        }
+        return;
+    }
+    for (int e0 = 0; e0 < ne; e0 += NT * B) {
+        T v[B];
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int e = e0 + i * NT + tid;
+            v[i] = (ok && e < ne) ? __hip_atomic_load(fg + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int e = e0 + i * NT + tid;
+            if (e < ne) {
+                const int k = e / (S * 6), r = e - k * (S * 6);
+                sh_k[r / 6][k][r % 6] = v[i];
+            }
+        }
+    }
+}
+
 // Folds wave-block outer forms w = 0..nw-1 (first row at [w][0..2], last row at
 // [w][3..5]) left to right into the outer form of the whole block.
 template <typename T>
@@ -180,11 +219,9 @@ __device__ __forceinline__ void contig_block_store(__amdgpu_buffer_rsrc_t rd, in
         tail_chunk_store<T, NT>(xb[q * 64 + lane], rd, wrow + (int64_t)(q * 64 + lane) * VV, len);
 }
 
-// Partial elimination over the K block forms of one system (blk[k][0..5]: first
-// row r, p, q then last row r, p, q; the system's ends are zero): returns
-// U = x before block j and V = x after it.  blk is overwritten.
+// block_chain (below) one block at a time: the form the FP64 kernels take.
 template <typename T>
-__device__ __forceinline__ void block_chain(T (*blk)[6], int K, int j, T &U, T &V) {
+__device__ __forceinline__ void block_chain_serial(T (*blk)[6], int K, int j, T &U, T &V) {
     T rho = 0, sig = 0;                   // x_{E_{k-1}} = rho - sig x_{F_k}
     for (int k = 0; k < K; ++k) {
         const T rF = blk[k][0], pF = blk[k][1], qF = blk[k][2];
@@ -206,6 +243,68 @@ __device__ __forceinline__ void block_chain(T (*blk)[6], int K, int j, T &U, T &
         if (k == j) U = blk[k][2] - blk[k][3] * xF;
         xFn = xF;
     }
+}
+
+// Partial elimination over the K block forms of one system (blk[k][0..5]: first
+// row r, p, q then last row r, p, q; the system's ends are zero): returns
+// U = x before block j and V = x after it.  blk is overwritten.
+template <typename T>
+__device__ __forceinline__ void block_chain(T (*blk)[6], int K, int j, T &U, T &V) {
+    if constexpr (sizeof(T) != 4) {
+        block_chain_serial<T>(blk, K, j, U, V);
+        return;
+    }
+    // FP32: both sweeps take the forms of four blocks from LDS before using the
+    // first, so that the LDS latency is paid once per four steps of the dependent
+    // chain (FP32 131,072 x 128 contiguous 0.088 -> 0.083 ms).  The FP64 kernels
+    // have no registers to spare and keep the step-by-step form.
+    constexpr int U4 = 4;
+    T rho = 0, sig = 0;                   // x_{E_{k-1}} = rho - sig x_{F_k}
+    auto fwd = [&](T (&f)[6], T *out) {
+        const T inv = rcp(T(1) - f[1] * sig);
+        const T rhoF = (f[0] - f[1] * rho) * inv;
+        const T sigF = f[2] * inv;
+        out[0] = rhoF; out[1] = sigF;
+        out[2] = rho; out[3] = sig;                    // relation of x_{E_{k-1}} to x_{F_k}
+        rho = f[3] - f[4] * (rho - sig * rhoF);
+        sig = f[5] + f[4] * sig * sigF;
+    };
+    int k = 0;
+    for (; k + U4 <= K; k += U4) {
+        T f[U4][6];
+#pragma unroll
+        for (int i = 0; i < U4; ++i)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) f[i][q] = blk[k + i][q];
+#pragma unroll
+        for (int i = 0; i < U4; ++i) fwd(f[i], blk[k + i]);
+    }
+    for (; k < K; ++k) {
+        T f[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) f[q] = blk[k][q];
+        fwd(f, blk[k]);
+    }
+    T xFn = 0;                            // x_{F_{k+1}}
+    U = 0;
+    V = 0;
+    auto bwd = [&](int kk, T rhoF, T sigF, T rin, T sin) {
+        if (kk == j) V = xFn;
+        const T xF = rhoF - sigF * xFn;
+        if (kk == j) U = rin - sin * xF;
+        xFn = xF;
+    };
+    k = K - 1;
+    for (; k - (U4 - 1) >= j; k -= U4) {
+        T b[U4][4];
+#pragma unroll
+        for (int i = 0; i < U4; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) b[i][q] = blk[k - i][q];
+#pragma unroll
+        for (int i = 0; i < U4; ++i) bwd(k - i, b[i][0], b[i][1], b[i][2], b[i][3]);
+    }
+    for (; k >= j; --k) bwd(k, blk[k][0], blk[k][1], blk[k][2], blk[k][3]);
 }
 
 // Contiguous blocks: whole 16-byte chunks of the wave's 64*M rows, issued
@@ -358,12 +457,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(OC
         }
         __syncthreads();
         const bool ok = s_ok != 0;
-        // the group's K block forms, staged in LDS by the whole workgroup
-        for (int e = tid; e < K * S * 6; e += 64 * NW) {
-            const int k = e / (S * 6), r = e - k * (S * 6);
-            sh_k[r / 6][k][r % 6] =
-                ok ? __hip_atomic_load(fg + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : T(0);
-        }
+        coop_stage_forms<T, S, NW, KMAX>(fg, K, ok, tid, sh_k);
         __syncthreads();
         if (tid < S) {
             T U, V;
@@ -421,11 +515,7 @@ __device__ __forceinline__ void coop_finish(const CoopParams<T> &p, int gi, int 
     }
     __syncthreads();
     const bool ok = *s_ok != 0;
-    for (int e = tid; e < K * S * 6; e += 64 * NW) {
-        const int k = e / (S * 6), r = e - k * (S * 6);
-        sh_k[r / 6][k][r % 6] =
-            ok ? __hip_atomic_load(fg + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : T(0);
-    }
+    coop_stage_forms<T, S, NW, KMAX>(fg, K, ok, tid, sh_k);
     __syncthreads();
     if (tid < S) {
         T U, V;
