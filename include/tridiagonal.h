@@ -167,8 +167,10 @@ int tridiag_solve_bulk(int precision, int algo, int layout, int64_t n_eqt, int64
  *     path tridiag_solve would take without it (its results then agree with
  *     tridiag_solve's to the solver's tolerance rather than bit for bit);
  *   - algorithm 0 (the vendor comparator, which allocates its own buffer) and
- *     interleaved systems longer than the interleaved tiles hold (solved through
- *     a contiguous copy of all four arrays) return TRIDIAG_EUNSUPPORTED.
+ *     interleaved systems that tridiag_solve solves through a contiguous copy of
+ *     all four arrays (above 4096 rows when the cooperative solve does not take
+ *     them: above 16384 rows, n not a multiple of its block rows, or its scratch
+ *     beyond the array) return TRIDIAG_EUNSUPPORTED.
  * Calls on one GPU share its array: calls that overlap on one GPU (another
  * stream, another thread) race.  The first call on a GPU looks up the array's
  * device address (hipGetSymbolAddress). */

@@ -40,8 +40,9 @@ def _assert_coop(prec, layout, n, batch, K):
 @pytest.mark.parametrize("dtype,n,batch,K", [
     (np.float32, 1024, 64, 4), (np.float32, 2048, 96, 8), (np.float32, 2560, 160, 10),
     (np.float32, 4096, 64, 16), (np.float32, 4096, 4096, 16), (np.float32, 6144, 31, 24),
-    (np.float32, 8192, 40, 32), (np.float64, 1280, 48, 5), (np.float64, 4096, 48, 16),
-    (np.float64, 4096, 4096, 16), (np.float64, 5120, 40, 20), (np.float64, 8192, 33, 32)])
+    (np.float32, 8192, 40, 32), (np.float32, 12288, 64, 48), (np.float64, 1280, 48, 5),
+    (np.float64, 4096, 48, 16), (np.float64, 4096, 4096, 16), (np.float64, 5120, 40, 20),
+    (np.float64, 8192, 33, 32), (np.float64, 16384, 21, 64)])
 def test_interleaved(dtype, n, batch, K):
     _assert_coop(0 if dtype == np.float32 else 1, 2, n, batch, K)
     announce("coop interleaved", np.dtype(dtype).name, n, batch)
@@ -90,7 +91,9 @@ def test_plan_boundaries():
     assert T.plan(0, -1, 2, 8192, 4001)["kind"] == "cooperative"
     assert T.plan(0, -1, 2, 2304, 64)["kind"] == "cooperative"
     assert T.plan(0, -1, 2, 2100, 64)["kind"] == "tile"             # blocks must divide n
-    assert T.plan(0, -1, 2, 8448, 64)["kind"] == "via_contiguous"   # 33 blocks
+    assert T.plan(0, -1, 2, 8448, 64)["kind"] == "cooperative"      # 33 blocks
+    assert T.plan(1, -1, 2, 16384, 64)["kind"] == "cooperative"     # 64 blocks
+    assert T.plan(0, -1, 2, 16640, 64)["kind"] == "via_contiguous"  # 65 blocks
     assert T.plan(1, -1, 0, 65536, 8)["kind"] == "cooperative"      # 32 blocks
     assert T.plan(1, -1, 0, 65537, 8)["kind"] == "chunked"          # 33 blocks
     assert T.plan(0, -1, 0, 131072, 8)["kind"] == "cooperative"

@@ -396,7 +396,8 @@ static int64_t env_int(const char *name, int64_t dflt) {
 // 0.078 ms).  At 4M unknowns FP32 1024 rows is a draw (0.028 vs 0.025 ms), FP64
 // 0.045 vs 0.054 ms.  Above 4096 rows (no interleaved tile) it replaces the solve
 // through a contiguous copy: 8192 rows 0.123 vs 0.201 ms (FP32), 0.257 vs 0.406 ms
-// (FP64).
+// (FP64); 16384 rows (64 blocks) 0.167 vs 0.213 ms (FP32), and FP64 12288 rows
+// 0.266 vs 0.363 ms.
 static int64_t coop_min_rows_il(int prec) {
     static const int64_t env = env_int("TRIDIAG_COOP_MIN_ROWS", 0);
     if (env != 0) return env;
@@ -449,8 +450,9 @@ static int32_t coop_spread(int groups_per_round) {
     return v != 0 ? groups_per_round : 0;
 }
 
-// The entry of the cooperative table for (layout, n): interleaved blocks must
-// divide n; K = blocks per system in [2, kmax].  Contiguous systems take it up to
+// The entry of the cooperative table for (layout, n): the first whose blocks
+// divide n (interleaved) and whose kmax holds K = blocks per system (K >= 2);
+// interleaved systems up to 64 blocks (16384 rows).  Contiguous systems take it up to
 // 32 blocks (FP32 131072 rows, FP64 65536): measured on MI355X at 2^24 unknowns
 // against the chunked solve, FP32 20000 rows 0.076 vs 0.113 ms and 131072 rows
 // 0.088 vs 0.107 ms, FP64 20000 rows 0.147 vs 0.209 ms and 65536 rows 0.173 vs

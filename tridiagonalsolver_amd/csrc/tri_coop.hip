@@ -48,8 +48,12 @@ static int coop_max_grid(bool pipelined) {
      &launch_coop<T, M, NW, S, IL, KMAX, PF, OCC>, &coop_max_grid<T, M, NW, S, IL, KMAX, PF, OCC>}
 
 // Interleaved: S systems = one 128-byte line per row, blocks of 2*NW*M (FP32) or
-// 4*NW*M (FP64) rows, up to 32 blocks.  Contiguous: one system, blocks of
-// 64*NW*M rows, up to 32 blocks.  PIPE: coop_pipe_kernel by default; PF: the
+// 4*NW*M (FP64) rows, up to 32 blocks, and an entry for 33-64 blocks whose larger
+// block-form staging leaves one workgroup per CU (interleaved 12288 / 16384 rows x
+// 1024: FP32 0.129 / 0.167 ms, FP64 0.266 / 0.367 ms, against 0.164 / 0.213 and
+// 0.363 / 0.477 ms through a contiguous copy).  Contiguous: one system, blocks of
+// 64*NW*M rows, up to 32 blocks (a group of 64 measured slower than the chunked
+// solve).  PIPE: coop_pipe_kernel by default; PF: the
 // next block in flight in registers; OCC: waves per SIMD.
 // Measured on MI355X (ms; 16.7 M unknowns unless noted):
 //   interleaved  FP32 4096 x 4096: PF, 1 workgroup per CU 0.117; pipelined 0.109;
@@ -66,10 +70,12 @@ static int coop_max_grid(bool pipelined) {
 // workgroups per CU keep more loads in flight, even where the kernel spills.
 static const CoopEntry<float> kCoopF32[] = {
     TRI_COOP_ENTRY(float, 16, 8, 32, true, 32, false, false, 4),
+    TRI_COOP_ENTRY(float, 16, 8, 32, true, 64, false, false, 4),
     TRI_COOP_ENTRY(float, 16, 4, 1, false, 32, false, false, 3),
 };
 static const CoopEntry<double> kCoopF64[] = {
     TRI_COOP_ENTRY(double, 8, 8, 16, true, 32, false, false, 4),
+    TRI_COOP_ENTRY(double, 8, 8, 16, true, 64, false, false, 4),
     TRI_COOP_ENTRY(double, 8, 4, 1, false, 32, false, false, 4),
 };
 #undef TRI_COOP_ENTRY

@@ -72,13 +72,13 @@ template <typename T> struct CoopParams {
     uint64_t tag;        // this solve's tag: different from every earlier solve's
 };
 
-// Publication flags: one 8-byte word per block of a group, the group's words in a
-// 256-byte line of their own (so that the polls of different groups go to
-// different memory channels).  A block is published when its word holds the
+// Publication flags: one 8-byte word per block of a group (up to 64 blocks), the
+// group's words in 512 bytes of their own (so that the polls of different groups
+// go to different memory channels).  A block is published when its word holds the
 // solve's tag.  Tags never repeat (a 64-bit sequence from a random start), so
 // words left by earlier solves, or never written, do not read as published and
 // the flags need no clearing between solves.
-constexpr int kCoopFlagStride = 32;
+constexpr int kCoopFlagStride = 64;
 // s_sleep argument between two polls of a wait (64 clocks per unit).
 #ifndef TRI_COOP_POLL_SLEEP
 #define TRI_COOP_POLL_SLEEP 8
@@ -372,7 +372,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(OC
     static_assert(IL || S == 1, "contiguous groups hold one system");
     static_assert(IL ? (S * (int)sizeof(T)) % 16 == 0 : M % (16 / (int)sizeof(T)) == 0,
                   "blocks move whole 16-byte chunks");
-    static_assert(KMAX <= kCoopFlagStride, "a group's flags fill at most one line");
+    static_assert(KMAX <= kCoopFlagStride, "a group's flags fit its flag words");
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int sl = IL ? lane % S : 0;        // system of the lane inside the group
     const int kw = IL ? lane / S : lane;     // segment inside the wave-block
