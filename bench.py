@@ -290,6 +290,9 @@ def run_workload(torch, T, name, dtname, layout, steps, warmup):
     ref_style = None
     if layout == 0:
         ref_style = reference_style_ms(torch, T, sets, n, batch, steps)
+    # where the allocator put b, c, d relative to a (MiB): the interleaved solve is
+    # sensitive to the spacing of the four arrays
+    spacing = [round((t.data_ptr() - sets[0][0].data_ptr()) / 2**20, 3) for t in sets[0][1:]]
     del sets
     torch.cuda.empty_cache()
     w = np.dtype(dtype).itemsize
@@ -300,7 +303,7 @@ def run_workload(torch, T, name, dtname, layout, steps, warmup):
     par = check_parity(torch, T, dtype, n, batch, -1, layout, 1e-5 if w == 4 else 1e-12)
     achieved = alg_bytes / (kern * 1e-3) / 1e9
     r = dict(ms_per_step=wall, kernel_ms=kern, unknowns_per_s=n * batch / (wall * 1e-3),
-             reference_style_sync_ms=ref_style,
+             reference_style_sync_ms=ref_style, array_spacing_mib=spacing,
              achieved_gbs=achieved, alg_bytes=alg_bytes, plan=pl, parity=par,
              layout={0: "contiguous", 2: "interleaved"}[layout],
              roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
