@@ -40,6 +40,8 @@ def _run(capi, fn, prec, algo, layout, n, batch, arrs):
     (0, 2, -1, [(4096, 640), (2048, 40)]),    # FP32 interleaved, cooperative blocks
     (1, 2, -1, [(4096, 320), (1024, 9)]),     # FP64 interleaved
     (0, 2, -1, [(4096, 2048), (2048, 3008)]),  # groups of a round spread over the batch
+    (0, 2, -1, [(16384, 96), (12288, 40)]),   # FP32 interleaved, 64 and 48 blocks
+    (1, 2, -1, [(16384, 48), (9216, 20)]),    # FP64 interleaved, 64 and 36 blocks
     (1, 0, 5, [(24581, 40), (9000, 5)]),      # PCR interface: the chunked solve
     (1, 0, -1, [(70000, 6), (66000, 3)]),     # FP64 beyond 32 blocks: the chunked solve
 ])
@@ -75,7 +77,8 @@ def test_allocating_paths_refused(lib):
     L = lib.lib()
     t = torch.zeros(5000 * 4, device="cuda")
     p = [ctypes.c_void_p(t.data_ptr())] * 4
-    # interleaved systems beyond the interleaved tiles go through a full copy
+    # interleaved systems beyond the interleaved tiles that the cooperative solve does
+    # not take (5000 rows: not a multiple of its blocks) go through a full copy
     assert L.tridiag_solve_noalloc(0, -1, 2, 5000, 4, *p, None) == lib.EUNSUPPORTED
     assert b"allocation" in L.tridiag_last_error()
     # the vendor comparator allocates its own buffer
