@@ -337,15 +337,18 @@ def run_pcr(torch, T, steps, warmup, algo=5):
     return {"algo": algo, "name": "reg PCR-pThomas", **out}
 
 
-BULK_IDS = [(3, "shmem PCR-pThomas"), (5, "reg PCR-pThomas"), (7, "shmem WM-pGE"),
-            (9, "reg WM-pGE")]
+# (id, name, words of memory traffic per unknown by design)
+BULK_IDS = [(2, "CR, levels in global memory", 23), (3, "shmem PCR-pThomas", 9),
+            (5, "reg PCR-pThomas", 9), (7, "shmem WM-pGE", 9), (9, "reg WM-pGE", 9)]
 
 
 def run_bulk(torch, T, steps, warmup):
     """The reference's own streaming forms of PCR-pThomas and WM-pGE
     (tridiag_solve_bulk: one wave per system, c' and d' through memory, 9 words
-    per unknown) at n=4096 x 4096 contiguous, FP32 and FP64, beside the partition
-    kernel tridiag_solve runs for the same ids."""
+    per unknown) and of CR (its reduced levels written and read back through
+    global memory: at most 23 words per unknown, 11.5 per row of each level) at
+    n=4096 x 4096 contiguous, FP32 and FP64, beside the partition kernel
+    tridiag_solve runs for the same ids."""
     import numpy as np
     n, batch = N_EQT, N_BATCH
     out = {}
@@ -355,17 +358,19 @@ def run_bulk(torch, T, steps, warmup):
         log("prepare bulk", name)
         sets = make_sets(torch, dtype, n, batch, N_SETS, seed0=11, layout=layout)
         res = {}
-        for algo, label in BULK_IDS:
+        for algo, label, words in BULK_IDS:
             wall, kern = time_solver(torch, T, sets, n, batch, steps, warmup, algo, layout,
                                      bulk=True)
             res[str(algo)] = dict(name=label, kernel_ms=kern, ms_per_step=wall,
                                   unknowns_per_s=n * batch / (wall * 1e-3),
                                   hbm_frac=round(5 * w * n * batch / (kern * 1e-3) / 1e9 /
                                                  HBM_PEAK_GBS, 4),
-                                  traffic_rate_gbs=round(9 * w * n * batch / (kern * 1e-3) / 1e9, 1))
+                                  traffic_words=words,
+                                  traffic_rate_gbs=round(words * w * n * batch / (kern * 1e-3) / 1e9,
+                                                         1))
         del sets
         torch.cuda.empty_cache()
-        for algo, _ in BULK_IDS:
+        for algo, _, _ in BULK_IDS:
             res[str(algo)]["parity"] = check_parity(torch, T, dtype, n, batch, algo, layout,
                                                     1e-5 if w == 4 else 1e-12, bulk=True)
         out[name] = res

@@ -134,9 +134,14 @@ int tridiag_algo_layout(int precision, int algo);
 int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch,
                   void *a, void *b, void *c, void *d, void *stream);
 
-/* The reference's own streaming form of algorithm ids 3-10, one wave per
- * system, with the reference's memory semantics (c' into c and d' into d on the
- * way down, x into d on the way up):
+/* The reference's own streaming form of algorithm ids 2-10, one wave per
+ * system, with the reference's memory semantics (ids 3-10: c' into c and d' into
+ * d on the way down, x into d on the way up; id 2: a, b, c kept, x into d):
+ *   2      CR (tridiagonal_basic.cu:32-164): cyclic reduction with its levels in
+ *          a stream-ordered scratch buffer of 4 * n_eqt * n_batch words (the odd
+ *          rows of each level kept) down to at most 64 rows, PCR across the
+ *          wave's lanes for those, then the even rows of every level on the way
+ *          back;
  *   3 / 5  PCR-pThomas (tridiagonal_wm_pcr.cu:886-1018, 1160-1282): 6 PCR steps
  *          over a sliding window of 64-row blocks, Thomas along the 64 stride-64
  *          subsystems;
@@ -150,7 +155,8 @@ int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_
  * on-chip partition kernel for these ids unless $TRIDIAG_FORM=bulk; this entry
  * always runs the bulk-row form.  Layouts 0 and 1; any n_eqt >= 1 (the
  * reference needs n % 64 for WM and n % 32 for PCR).  Other ids and the
- * interleaved layout return TRIDIAG_EUNSUPPORTED.  Asynchronous on `stream`. */
+ * interleaved layout return TRIDIAG_EUNSUPPORTED.  Asynchronous on `stream`.
+ * tridiag_solve_noalloc refuses id 2 under $TRIDIAG_FORM=bulk (its scratch). */
 int tridiag_solve_bulk(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch,
                        void *a, void *b, void *c, void *d, void *stream);
 

@@ -1,9 +1,11 @@
-"""GPU parity of the bulk-row forms of the reference's ids 3-10 (tridiag_solve_bulk:
+"""GPU parity of the bulk-row forms of the reference's ids 2-10 (tridiag_solve_bulk:
 PCR-pThomas and WM-pGE streamed through one wave per system, full and half forms,
-LDS or ds_bpermute exchanges), through the C-ABI, against the FP64 oracle:
+LDS or ds_bpermute exchanges; CR with its levels in global memory), through the
+C-ABI, against the FP64 oracle:
 normwise relative error <= 1e-5 (FP32) / 1e-12 (FP64).  Sizes straddle the
 64-row blocks and 128-row chunks and the prefetch rings (4 blocks forward, 8
-back), and include sizes the reference itself does not take (n % 64 != 0)."""
+back), CR's odd and even level lengths and its 64-row last level, and include
+sizes the reference itself does not take (n % 64 != 0)."""
 import os
 import subprocess
 import sys
@@ -17,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DT = {0: np.float32, 1: np.float64}
-ALGOS = [3, 4, 5, 6, 7, 8, 9, 10]
+ALGOS = [2, 3, 4, 5, 6, 7, 8, 9, 10]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -53,7 +55,7 @@ def test_bulk_sizes(prec, algo):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-@pytest.mark.parametrize("algo", [5, 6, 9, 10])
+@pytest.mark.parametrize("algo", [2, 5, 6, 9, 10])
 def test_bulk_low_margin(prec, algo):
     n, batch = 4096, 20
     announce("bulk low margin", prec, algo)
@@ -63,7 +65,7 @@ def test_bulk_low_margin(prec, algo):
     assert st["nonfinite"] == 0 and st["relerr"] <= TOL[DT[prec]], st
 
 
-@pytest.mark.parametrize("algo", [3, 9])
+@pytest.mark.parametrize("algo", [2, 3, 9])
 def test_bulk_packed_layout(algo):
     n, batch = 1000, 33
     announce("bulk packed", algo)
@@ -79,7 +81,7 @@ def test_bulk_refuses_other_ids_and_layouts():
     from tridiagonalsolver_amd import _capi
     L = T.lib()
     nul = ctypes.c_void_p(0)
-    for algo in (0, 1, 2, 11, 12, 13):
+    for algo in (0, 1, 11, 12, 13):
         assert L.tridiag_solve_bulk(0, algo, 0, 64, 1, nul, nul, nul, nul, nul) == \
             _capi.EUNSUPPORTED, algo
     assert L.tridiag_solve_bulk(0, 9, 2, 64, 1, nul, nul, nul, nul, nul) == _capi.EUNSUPPORTED
@@ -92,21 +94,29 @@ sys.path.insert(0, %(root)r)
 sys.path.insert(0, %(root)r + '/tests')
 import tridiagonalsolver_amd as T
 from gpu_util import TOL, check_against_oracle, make_case, run_solve
-for prec, algo in ((0, 5), (1, 9), (0, 8), (1, 4)):
+for prec, algo in ((0, 5), (1, 9), (0, 8), (1, 4), (0, 2), (1, 2)):
     dtype = np.float32 if prec == 0 else np.float64
     assert T.plan(prec, algo, 0, 4096, 16)["kind"] == "bulk_row"
-    assert T.plan(prec, 2, 0, 4096, 16)["kind"] == "tile"
+    assert T.plan(prec, 11, 0, 4096, 16)["kind"] == "tile"
     assert T.plan(prec, algo, 2, 4096, 16)["kind"] != "bulk_row"
     arrs = make_case(4096, 16, 0, dtype, seed=algo)
     x = run_solve(arrs, 4096, 16, 0, algo, dtype)
     st, _ = check_against_oracle(arrs, x, 4096, 16, 0, dtype)
     assert st["relerr"] <= TOL[dtype], st
+# the CR form needs its scratch: the no-allocation entry refuses it
+import ctypes, torch
+from tridiagonalsolver_amd import _capi
+t = torch.zeros(4 * 4096, device="cuda")
+p = [ctypes.c_void_p(t.data_ptr())] * 4
+assert T.lib().tridiag_solve_noalloc(0, 2, 0, 4096, 1, *p, None) == _capi.EUNSUPPORTED
+assert T.lib().tridiag_solve_noalloc(0, 9, 0, 4096, 1, *p, None) == _capi.OK
+torch.cuda.synchronize()
 print("ok", flush=True)
 """
 
 
 def test_form_env_routes_reference_ids():
-    """TRIDIAG_FORM=bulk: tridiag_solve (and the reference symbols) run ids 3-10 in
+    """TRIDIAG_FORM=bulk: tridiag_solve (and the reference symbols) run ids 2-10 in
     their bulk-row forms in layouts 0 and 1 (child process: read once)."""
     env = dict(os.environ, TRIDIAG_FORM="bulk")
     announce("bulk routing (child process)")

@@ -3,7 +3,8 @@
 // semantics (c' into c and d' into d on the way down, x into d on the way up:
 // 9 words per unknown).  These are the algorithms of the reference's ids 3-10
 // themselves, for callers that want them rather than the partition kernel that
-// tridiag_solve runs for those ids by default.
+// tridiag_solve runs for those ids by default; id 2, cyclic reduction with its
+// levels in global memory, is at the end.
 //
 //   WM-pGE (Tridiagonal_CUDA/tridiagonal_wm_pcr.cu:17-205, 424-646; WM-half-pGE
 //   :207-422, 648-884).  A chunk is 128 rows: lane t holds rows t and 64 + t.
@@ -389,6 +390,146 @@ __global__ void __launch_bounds__(256) bulk_pcr_kernel(const T *a, const T *b, T
     }
 }
 
+// ---------------------------------------------------------------- CR through global memory
+// Cyclic reduction with its levels in global memory (Tridiagonal_CUDA/
+// tridiagonal_basic.cu:32-164, the reference's CR id): one wave per system.
+// Level l + 1 keeps the odd rows j = 2i + 1 of level l and eliminates the even
+// rows around them,
+//     k1 = a_j / b_(j-1),  k2 = c_j / b_(j+1),
+//     a'_i = -a_(j-1) k1,  c'_i = -c_(j+1) k2,
+//     b'_i = b_j - c_(j-1) k1 - a_(j+1) k2,  d'_i = d_j - d_(j-1) k1 - d_(j+1) k2,
+// (row j + 1 taken as an identity row past the level's end), until at most 64
+// rows are left.  Those are solved by PCR across the wave's lanes (strides 1 to
+// 32), and the way back fills in the even rows of every level,
+//     x_j = (d_j - a_j x_(j-1) - c_j x_(j+1)) / b_j,
+// the odd rows taking x of the level above.  Levels 1 and up sit in a scratch
+// buffer of 4n words per system (for level l >= 1, rows at offset
+// n/2 + n/4 + ... of the l - 1 levels before it in each of four n-word arrays);
+// x of a level replaces its d.  a, b and c keep their values; x replaces d.
+template <typename T> struct CrRow {
+    T a, b, c, d;
+};
+
+template <typename T> struct CrLevels {
+    BulkIO<T> io;
+    T *sa, *sb, *sc, *sd;       // this system's scratch arrays (n words each)
+    // row i of the level at scratch offset `off` (level 0: the input, with
+    // a_0 and c_(n-1) dropped); rows outside [0, len) are identity rows
+    __device__ __forceinline__ CrRow<T> row(int lev, int64_t off, int64_t len, int64_t i) const {
+        CrRow<T> r{T(0), T(1), T(0), T(0)};
+        if (i < 0 || i >= len) return r;
+        if (lev == 0) {
+            r.a = io.load(0, i);
+            r.b = io.load(1, i);
+            r.c = io.load(2, i);
+            r.d = io.load(3, i);
+            io.fix(i, r.a, r.b, r.c, r.d);
+        } else {
+            r.a = sa[off + i];
+            r.b = sb[off + i];
+            r.c = sc[off + i];
+            r.d = sd[off + i];
+        }
+        return r;
+    }
+    __device__ __forceinline__ T x(int64_t off, int64_t i) const { return sd[off + i]; }
+    __device__ __forceinline__ void put_x(int lev, int64_t off, int64_t i, T v) const {
+        if (lev == 0) io.store(3, i, v);
+        else sd[off + i] = v;
+    }
+};
+
+// Orders the global-memory writes of the wave's lanes before their later reads.
+__device__ __forceinline__ void wave_global_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// rows and scratch offset of level `lev` (level 0: n rows, the input)
+__device__ __forceinline__ void cr_level(int64_t n, int lev, int64_t &len, int64_t &off) {
+    len = n;
+    off = 0;
+    for (int l = 1; l <= lev; ++l) {
+        if (l >= 2) off += len;
+        len /= 2;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) bulk_cr_kernel(const T *a, const T *b, T *c, T *d,
+                                                      int64_t n, int64_t batch, int64_t sstride,
+                                                      int32_t bb, T *scratch) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t s = (int64_t)blockIdx.x * 4 + wave;
+    if (s >= batch) return;                   // whole waves; no workgroup barriers
+    T *const sys = scratch + s * 4 * n;
+    const CrLevels<T> L{{{make_rsrc(a, bb), make_rsrc(b, bb), make_rsrc(c, bb), make_rsrc(d, bb)},
+                         s * sstride, n},
+                        sys, sys + n, sys + 2 * n, sys + 3 * n};
+
+    // forward: reduce to at most 64 rows
+    int lev = 0;
+    int64_t len = n, off = 0;
+    while (len > 64) {
+        const int64_t nlen = len / 2, noff = lev == 0 ? 0 : off + len;
+        for (int64_t i0 = 0; i0 < nlen; i0 += 64) {
+            const int64_t i = i0 + lane;
+            if (i < nlen) {
+                const int64_t j = 2 * i + 1;
+                const CrRow<T> p = L.row(lev, off, len, j - 1), m = L.row(lev, off, len, j),
+                               q = L.row(lev, off, len, j + 1);
+                const T k1 = m.a * rcp(p.b), k2 = m.c * rcp(q.b);
+                L.sa[noff + i] = -p.a * k1;
+                L.sb[noff + i] = m.b - p.c * k1 - q.a * k2;
+                L.sc[noff + i] = -q.c * k2;
+                L.sd[noff + i] = m.d - p.d * k1 - q.d * k2;
+            }
+        }
+        wave_global_sync();
+        len = nlen;
+        off = noff;
+        ++lev;
+    }
+
+    // the last level (at most 64 rows, one per lane): PCR across the wave
+    CrRow<T> r = L.row(lev, off, len, lane);
+    for (int st = 1; st < 64; st <<= 1) {
+        // every lane reads (ds_bpermute takes its data from active lanes only),
+        // then rows outside the wave become identity rows
+        const bool hp = lane >= st, hq = lane + st < 64;
+        const int sp = hp ? lane - st : lane, sq = hq ? lane + st : lane;
+        CrRow<T> p{lane_read(r.a, sp), lane_read(r.b, sp), lane_read(r.c, sp), lane_read(r.d, sp)};
+        CrRow<T> q{lane_read(r.a, sq), lane_read(r.b, sq), lane_read(r.c, sq), lane_read(r.d, sq)};
+        if (!hp) p = CrRow<T>{T(0), T(1), T(0), T(0)};
+        if (!hq) q = CrRow<T>{T(0), T(1), T(0), T(0)};
+        const T k1 = r.a * rcp(p.b), k2 = r.c * rcp(q.b);
+        r = CrRow<T>{-p.a * k1, r.b - p.c * k1 - q.a * k2, -q.c * k2, r.d - p.d * k1 - q.d * k2};
+    }
+    if (lane < len) L.put_x(lev, off, lane, r.d * rcp(r.b));
+    wave_global_sync();
+
+    // back: the even rows of every level from the odd rows' x one level up
+    for (int l = lev - 1; l >= 0; --l) {
+        int64_t ln, lo, un, uo;
+        cr_level(n, l, ln, lo);
+        cr_level(n, l + 1, un, uo);
+        const int64_t ne = (ln + 1) / 2;
+        for (int64_t i0 = 0; i0 < ne; i0 += 64) {
+            const int64_t i = i0 + lane;
+            if (i < ne) {
+                const int64_t j = 2 * i;
+                const T xl = i >= 1 ? L.x(uo, i - 1) : T(0);
+                const T xr = j + 1 < ln ? L.x(uo, i) : T(0);
+                const CrRow<T> m = L.row(l, lo, ln, j);
+                L.put_x(l, lo, j, (m.d - m.a * xl - m.c * xr) * rcp(m.b));
+                if (j + 1 < ln) L.put_x(l, lo, j + 1, xr);
+            }
+        }
+        wave_global_sync();
+    }
+}
+
 // ---------------------------------------------------------------- launchers
 template <typename T, class K>
 static hipError_t bulk_launch(K kernel, const T *a, const T *b, T *c, T *d, int64_t n,
@@ -420,6 +561,29 @@ static hipError_t launch_bulk(int form, bool lds, const T *a, const T *b, T *c, 
     default: return hipErrorInvalidValue;
     }
 #undef TRI_BULK
+}
+
+// CR through global memory; scratch: 4 * n * batch words.
+template <typename T>
+static hipError_t launch_bulk_cr(const T *a, const T *b, T *c, T *d, int64_t n, int64_t batch,
+                                 int64_t sstride, int32_t bb, T *scratch, hipStream_t st) {
+    if (n < 1 || batch < 0 || bb <= 0 || sstride < n || !scratch) return hipErrorInvalidValue;
+    if (batch == 0) return hipSuccess;
+    const int64_t wg = (batch + 3) / 4;
+    if (wg > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((bulk_cr_kernel<T>), dim3((unsigned)wg), dim3(256), 0, st, a, b, c, d, n,
+                       batch, sstride, bb, scratch);
+    return hipGetLastError();
+}
+hipError_t launch_bulk_cr_f32(const float *a, const float *b, float *c, float *d, int64_t n,
+                              int64_t batch, int64_t sstride, int32_t bb, float *scratch,
+                              hipStream_t st) {
+    return launch_bulk_cr<float>(a, b, c, d, n, batch, sstride, bb, scratch, st);
+}
+hipError_t launch_bulk_cr_f64(const double *a, const double *b, double *c, double *d, int64_t n,
+                              int64_t batch, int64_t sstride, int32_t bb, double *scratch,
+                              hipStream_t st) {
+    return launch_bulk_cr<double>(a, b, c, d, n, batch, sstride, bb, scratch, st);
 }
 
 hipError_t launch_bulk_f32(int form, bool lds, const float *a, const float *b, float *c, float *d,

@@ -475,11 +475,12 @@ static const CoopEntry<T> *choose_coop(bool il, int64_t n, int &K) {
 }
 
 // Bulk-row form of a reference id (tri_bulk.hip): form 0 PCR-pThomas, 1
-// PCR-half-pThomas, 2 WM-pGE, 3 WM-half-pGE, and whether its exchanges go through
-// LDS (the shared-memory ids) or ds_bpermute (the register ids).  False for ids
-// without one.
+// PCR-half-pThomas, 2 WM-pGE, 3 WM-half-pGE, 4 CR through global memory, and
+// whether its exchanges go through LDS (the shared-memory ids) or ds_bpermute
+// (the register ids).  False for ids without one.
 static bool bulk_form(int algo, int &form, bool &lds) {
     switch (algo) {
+    case kCR: form = 4; lds = false; return true;
     case kPCR5Shmem: form = 0; lds = true; return true;
     case kPCR4Shmem: form = 1; lds = true; return true;
     case kPCR5Reg: form = 0; lds = false; return true;
@@ -491,7 +492,7 @@ static bool bulk_form(int algo, int &form, bool &lds) {
     default: return false;
     }
 }
-// $TRIDIAG_FORM=bulk: ids 3-10 in the contiguous and packed layouts run their
+// $TRIDIAG_FORM=bulk: ids 2-10 in the contiguous and packed layouts run their
 // bulk-row forms (also through the reference symbols) instead of the partition
 // kernel.
 static bool bulk_forms_requested() {
@@ -501,12 +502,28 @@ static bool bulk_forms_requested() {
     }();
     return v;
 }
+// The CR form keeps its reduced levels in a stream-ordered scratch buffer of
+// 4 * n * batch words, freed after the launch.
 template <typename T>
 static hipError_t bulk_launch(int algo, const T *a, const T *b, T *c, T *d, int64_t n,
                               int64_t batch, int64_t sstride, int32_t bb, hipStream_t st) {
     int form = 0;
     bool lds = false;
     if (!bulk_form(algo, form, lds)) return hipErrorInvalidValue;
+    if (form == 4) {
+        if (batch == 0) return hipSuccess;
+        void *scratch = nullptr;
+        hipError_t e = hipMallocAsync(&scratch, (size_t)(4 * n * batch) * sizeof(T), st);
+        if (e != hipSuccess) return e;
+        if constexpr (sizeof(T) == 4)
+            e = launch_bulk_cr_f32((const float *)a, (const float *)b, (float *)c, (float *)d, n,
+                                   batch, sstride, bb, (float *)scratch, st);
+        else
+            e = launch_bulk_cr_f64((const double *)a, (const double *)b, (double *)c,
+                                   (double *)d, n, batch, sstride, bb, (double *)scratch, st);
+        const hipError_t ef = hipFreeAsync(scratch, st);
+        return e != hipSuccess ? e : ef;
+    }
     if constexpr (sizeof(T) == 4)
         return launch_bulk_f32(form, lds, (const float *)a, (const float *)b, (float *)c,
                                (float *)d, n, batch, sstride, bb, st);
@@ -523,7 +540,7 @@ enum PlanKind {
     kPlanCoop = 3,           // on chip, K workgroups per group of systems
     kPlanChunked = 4,        // three passes over chunks through global memory
     kPlanViaContiguous = 5,  // interleaved systems solved through a contiguous copy
-    kPlanBulk = 6,           // the reference's bulk-row form of ids 3-10, one wave per system
+    kPlanBulk = 6,           // the reference's bulk-row form of ids 2-10, one wave per system
 };
 template <typename T> struct SolvePlan {
     int kind = kPlanTile;
@@ -761,6 +778,9 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
         return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "Thomas launch");
     }
     case kPlanBulk: {
+        if (fx && algo == kCR)
+            return fail(TRIDIAG_EUNSUPPORTED, "the global-memory CR form keeps its levels in "
+                                              "an allocated buffer");
         const hipError_t e = bulk_launch<T>(algo, a, b, c, d, n, batch, sstride, bb, st);
         return e == hipSuccess ? TRIDIAG_OK : hip_fail(e, "bulk-row launch");
     }
@@ -787,7 +807,7 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
     return err == hipSuccess ? TRIDIAG_OK : hip_fail(err, "tile launch");
 }
 
-// The bulk-row form of algorithm `algo` (ids 3-10) in the contiguous or packed
+// The bulk-row form of algorithm `algo` (ids 2-10) in the contiguous or packed
 // layout; batches whose arrays reach 2 GiB run in sub-2-GiB launches.
 template <typename T>
 static int bulk_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b, T *c, T *d,
@@ -796,7 +816,7 @@ static int bulk_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b,
     bool lds = false;
     if (!bulk_form(algo, form, lds))
         return fail(TRIDIAG_EUNSUPPORTED, "algorithm " + std::to_string(algo) +
-                                              " has no bulk-row form (ids 3-10 do)");
+                                              " has no bulk-row form (ids 2-10 do)");
     if (n < 1) return fail(TRIDIAG_EINVAL, "n_eqt must be >= 1");
     if (batch < 0) return fail(TRIDIAG_EINVAL, "n_batch must be >= 0");
     if (layout == kInterleaved)

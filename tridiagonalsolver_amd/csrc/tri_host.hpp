@@ -126,6 +126,14 @@ hipError_t launch_bulk_f32(int form, bool lds, const float *a, const float *b, f
 hipError_t launch_bulk_f64(int form, bool lds, const double *a, const double *b, double *c,
                            double *d, int64_t n, int64_t batch, int64_t sstride, int32_t bb,
                            hipStream_t st);
+// The reference's CR id through global memory (tri_bulk.hip), one wave per system;
+// scratch holds 4 * n * batch words (the reduced levels).
+hipError_t launch_bulk_cr_f32(const float *a, const float *b, float *c, float *d, int64_t n,
+                              int64_t batch, int64_t sstride, int32_t bb, float *scratch,
+                              hipStream_t st);
+hipError_t launch_bulk_cr_f64(const double *a, const double *b, double *c, double *d, int64_t n,
+                              int64_t batch, int64_t sstride, int32_t bb, double *scratch,
+                              hipStream_t st);
 // Composed adjoint (systems the fused adjoint kernels do not hold): the shifted
 // coefficients of A^T into ap / cp, g into lam, and d into xo when d is given;
 // then, after the solves, the coefficient gradients from x and lambda.
