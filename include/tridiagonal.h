@@ -134,9 +134,9 @@ int tridiag_algo_layout(int precision, int algo);
 int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch,
                   void *a, void *b, void *c, void *d, void *stream);
 
-/* The reference's own streaming form of algorithm ids 2-10, one wave per
- * system, with the reference's memory semantics (ids 3-10: c' into c and d' into
- * d on the way down, x into d on the way up; id 2: a, b, c kept, x into d):
+/* The reference's own form of algorithm ids 2-13 (ids 2-10 one wave per system),
+ * with the reference's memory semantics (ids 3-10: c' into c and d' into d on the
+ * way down, x into d on the way up; ids 2, 11-13: a, b, c kept, x into d):
  *   2      CR (tridiagonal_basic.cu:32-164): cyclic reduction with its levels in
  *          a stream-ordered scratch buffer of 4 * n_eqt * n_batch words (the odd
  *          rows of each level kept) down to at most 64 rows, PCR across the
@@ -150,6 +150,11 @@ int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_
  *   7 / 9  WM-pGE (:17-205, 424-646): 128-row chunks, 6 Wang-Mou merge rounds to
  *          64-row blocks, partial Gaussian elimination block after block;
  *   8 / 10 WM-half-pGE (:207-422, 648-884): 5 rounds, 32-row blocks;
+ *   11-13  CR_reg4/8/16 (tridiagonal_onchip.cu:170-369): one workgroup per
+ *          system, R = 4, 8, 16 rows per lane reduced in registers, PCR in LDS
+ *          over the lanes' last rows, then back; n_eqt <= 1024 R (longer systems
+ *          return TRIDIAG_EUNSUPPORTED here and keep the default path under
+ *          $TRIDIAG_FORM=bulk);
  * the shared-memory ids (3, 4, 7, 8) exchange rows between lanes through LDS,
  * the register ids (5, 6, 9, 10) through ds_bpermute.  tridiag_solve runs the
  * on-chip partition kernel for these ids unless $TRIDIAG_FORM=bulk; this entry

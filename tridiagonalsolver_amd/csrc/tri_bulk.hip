@@ -530,6 +530,108 @@ __global__ void __launch_bounds__(256) bulk_cr_kernel(const T *a, const T *b, T 
     }
 }
 
+// ---------------------------------------------------------------- CR over R rows per lane
+// The reference's register CR (Tridiagonal_CUDA/tridiagonal_onchip.cu:170-369,
+// CR_reg4/8/16): one workgroup per system, lane t holding rows tR .. tR + R - 1 in
+// registers.  log2 R cyclic-reduction levels run inside the lanes: at level k
+// (h = 2^k) the rows r with (r + 1) % 2h == 0 take in their neighbours r - h and
+// r + h (the next lane's row h - 1 for the lane's last row, through LDS), so that
+// each lane's last row couples only to the last rows of the lanes beside it.
+// That system of one row per lane is solved by PCR in LDS (the reference runs CR
+// there), and the eliminated rows are filled in level by level on the way back,
+// the previous lane's last x standing in for row -1.  Any n up to 1024 R (the
+// reference's bound); rows past n are identity rows.  a, b, c keep their values;
+// x replaces d.
+template <typename T, int R>
+__global__ void __launch_bounds__(1024) bulk_crreg_kernel(const T *a, const T *b, T *c, T *d,
+                                                          int64_t n, int64_t batch,
+                                                          int64_t sstride, int32_t bb) {
+    static_assert(R >= 2 && (R & (R - 1)) == 0, "R is a power of two");
+    constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : R == 16 ? 4 : 5;
+    __shared__ T sh[4][1024];
+    const int t = threadIdx.x, P = blockDim.x;
+    const int64_t s = blockIdx.x;
+    const BulkIO<T> io{{make_rsrc(a, bb), make_rsrc(b, bb), make_rsrc(c, bb), make_rsrc(d, bb)},
+                       s * sstride, n};
+    T ra[R], rb[R], rc[R], rd[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int64_t i = (int64_t)t * R + r;
+        ra[r] = io.load(0, i);
+        rb[r] = io.load(1, i);
+        rc[r] = io.load(2, i);
+        rd[r] = io.load(3, i);
+        io.fix(i, ra[r], rb[r], rc[r], rd[r]);
+    }
+
+    // reduction inside the lanes
+#pragma unroll
+    for (int k = 0; k < LR; ++k) {
+        const int h = 1 << k;
+        __syncthreads();
+        sh[0][t] = ra[h - 1];
+        sh[1][t] = rb[h - 1];
+        sh[2][t] = rc[h - 1];
+        sh[3][t] = rd[h - 1];
+        __syncthreads();
+        const bool last = t == P - 1;
+        const T na = last ? T(0) : sh[0][t + 1], nb = last ? T(1) : sh[1][t + 1];
+        const T nc = last ? T(0) : sh[2][t + 1], nd = last ? T(0) : sh[3][t + 1];
+#pragma unroll
+        for (int r = 2 * h - 1; r < R; r += 2 * h) {
+            const int p = r - h;
+            const T qa = r + h < R ? ra[r + h < R ? r + h : 0] : na;
+            const T qb = r + h < R ? rb[r + h < R ? r + h : 0] : nb;
+            const T qc = r + h < R ? rc[r + h < R ? r + h : 0] : nc;
+            const T qd = r + h < R ? rd[r + h < R ? r + h : 0] : nd;
+            const T k1 = ra[r] * rcp(rb[p]), k2 = rc[r] * rcp(qb);
+            ra[r] = -ra[p] * k1;
+            rb[r] = rb[r] - rc[p] * k1 - qa * k2;
+            rc[r] = -qc * k2;
+            rd[r] = rd[r] - rd[p] * k1 - qd * k2;
+        }
+    }
+
+    // the lanes' last rows: PCR across the workgroup in LDS
+    T ia = ra[R - 1], ib = rb[R - 1], ic = rc[R - 1], id = rd[R - 1];
+    for (int st = 1; st < P; st <<= 1) {
+        __syncthreads();
+        sh[0][t] = ia;
+        sh[1][t] = ib;
+        sh[2][t] = ic;
+        sh[3][t] = id;
+        __syncthreads();
+        const bool hp = t >= st, hq = t + st < P;
+        const T pa = hp ? sh[0][t - st] : T(0), pb = hp ? sh[1][t - st] : T(1);
+        const T pc = hp ? sh[2][t - st] : T(0), pd = hp ? sh[3][t - st] : T(0);
+        const T qa = hq ? sh[0][t + st] : T(0), qb = hq ? sh[1][t + st] : T(1);
+        const T qc = hq ? sh[2][t + st] : T(0), qd = hq ? sh[3][t + st] : T(0);
+        const T k1 = ia * rcp(pb), k2 = ic * rcp(qb);
+        ia = -pa * k1;
+        ib = ib - pc * k1 - qa * k2;
+        ic = -qc * k2;
+        id = id - pd * k1 - qd * k2;
+    }
+    rd[R - 1] = id * rcp(ib);
+    __syncthreads();
+    sh[0][t] = rd[R - 1];
+    __syncthreads();
+    const T xprev = t > 0 ? sh[0][t - 1] : T(0);
+
+    // back inside the lanes: the rows eliminated at level k, last level first
+#pragma unroll
+    for (int k = LR - 1; k >= 0; --k) {
+        const int h = 1 << k;
+#pragma unroll
+        for (int r = h - 1; r < R; r += 2 * h) {
+            const T xl = r >= h ? rd[r >= h ? r - h : 0] : xprev;
+            rd[r] = (rd[r] - ra[r] * xl - rc[r] * rd[r + h]) * rcp(rb[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) io.store(3, (int64_t)t * R + r, rd[r]);
+}
+
 // ---------------------------------------------------------------- launchers
 template <typename T, class K>
 static hipError_t bulk_launch(K kernel, const T *a, const T *b, T *c, T *d, int64_t n,
@@ -584,6 +686,46 @@ hipError_t launch_bulk_cr_f64(const double *a, const double *b, double *c, doubl
                               int64_t batch, int64_t sstride, int32_t bb, double *scratch,
                               hipStream_t st) {
     return launch_bulk_cr<double>(a, b, c, d, n, batch, sstride, bb, scratch, st);
+}
+
+// Register CR over R = 4, 8 or 16 rows per lane, one workgroup per system of at
+// most 1024 R rows.
+template <typename T>
+static hipError_t launch_bulk_crreg(int R, const T *a, const T *b, T *c, T *d, int64_t n,
+                                    int64_t batch, int64_t sstride, int32_t bb, hipStream_t st) {
+    if (n < 1 || batch < 0 || bb <= 0 || sstride < n || n > 1024 * (int64_t)R)
+        return hipErrorInvalidValue;
+    if (batch == 0) return hipSuccess;
+    if (batch > 0x7fffffff) return hipErrorInvalidValue;
+    const int64_t lanes = (n + R - 1) / R;
+    const unsigned P = (unsigned)((lanes + 63) / 64 * 64);
+    switch (R) {
+    case 4:
+        hipLaunchKernelGGL((bulk_crreg_kernel<T, 4>), dim3((unsigned)batch), dim3(P), 0, st, a, b, c,
+                           d, n, batch, sstride, bb);
+        break;
+    case 8:
+        hipLaunchKernelGGL((bulk_crreg_kernel<T, 8>), dim3((unsigned)batch), dim3(P), 0, st, a, b, c,
+                           d, n, batch, sstride, bb);
+        break;
+    case 16:
+        hipLaunchKernelGGL((bulk_crreg_kernel<T, 16>), dim3((unsigned)batch), dim3(P), 0, st, a, b,
+                           c, d, n, batch, sstride, bb);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+hipError_t launch_bulk_crreg_f32(int R, const float *a, const float *b, float *c, float *d,
+                                 int64_t n, int64_t batch, int64_t sstride, int32_t bb,
+                                 hipStream_t st) {
+    return launch_bulk_crreg<float>(R, a, b, c, d, n, batch, sstride, bb, st);
+}
+hipError_t launch_bulk_crreg_f64(int R, const double *a, const double *b, double *c, double *d,
+                                 int64_t n, int64_t batch, int64_t sstride, int32_t bb,
+                                 hipStream_t st) {
+    return launch_bulk_crreg<double>(R, a, b, c, d, n, batch, sstride, bb, st);
 }
 
 hipError_t launch_bulk_f32(int form, bool lds, const float *a, const float *b, float *c, float *d,

@@ -475,12 +475,14 @@ static const CoopEntry<T> *choose_coop(bool il, int64_t n, int &K) {
 }
 
 // Bulk-row form of a reference id (tri_bulk.hip): form 0 PCR-pThomas, 1
-// PCR-half-pThomas, 2 WM-pGE, 3 WM-half-pGE, 4 CR through global memory, and
-// whether its exchanges go through LDS (the shared-memory ids) or ds_bpermute
-// (the register ids).  False for ids without one.
+// PCR-half-pThomas, 2 WM-pGE, 3 WM-half-pGE, 4 CR through global memory, 5 CR over
+// R rows per lane (R = forced_M(algo)), and whether its exchanges go through LDS
+// (the shared-memory ids) or ds_bpermute (the register ids).  False for ids
+// without one.
 static bool bulk_form(int algo, int &form, bool &lds) {
     switch (algo) {
     case kCR: form = 4; lds = false; return true;
+    case kCRReg4: case kCRReg8: case kCRReg16: form = 5; lds = true; return true;
     case kPCR5Shmem: form = 0; lds = true; return true;
     case kPCR4Shmem: form = 1; lds = true; return true;
     case kPCR5Reg: form = 0; lds = false; return true;
@@ -492,7 +494,7 @@ static bool bulk_form(int algo, int &form, bool &lds) {
     default: return false;
     }
 }
-// $TRIDIAG_FORM=bulk: ids 2-10 in the contiguous and packed layouts run their
+// $TRIDIAG_FORM=bulk: ids 2-13 in the contiguous and packed layouts run their
 // bulk-row forms (also through the reference symbols) instead of the partition
 // kernel.
 static bool bulk_forms_requested() {
@@ -510,6 +512,14 @@ static hipError_t bulk_launch(int algo, const T *a, const T *b, T *c, T *d, int6
     int form = 0;
     bool lds = false;
     if (!bulk_form(algo, form, lds)) return hipErrorInvalidValue;
+    if (form == 5) {
+        if constexpr (sizeof(T) == 4)
+            return launch_bulk_crreg_f32(forced_M(algo), (const float *)a, (const float *)b,
+                                         (float *)c, (float *)d, n, batch, sstride, bb, st);
+        else
+            return launch_bulk_crreg_f64(forced_M(algo), (const double *)a, (const double *)b,
+                                         (double *)c, (double *)d, n, batch, sstride, bb, st);
+    }
     if (form == 4) {
         if (batch == 0) return hipSuccess;
         void *scratch = nullptr;
@@ -540,7 +550,7 @@ enum PlanKind {
     kPlanCoop = 3,           // on chip, K workgroups per group of systems
     kPlanChunked = 4,        // three passes over chunks through global memory
     kPlanViaContiguous = 5,  // interleaved systems solved through a contiguous copy
-    kPlanBulk = 6,           // the reference's bulk-row form of ids 2-10, one wave per system
+    kPlanBulk = 6,           // the reference's own form of ids 2-13 (bulk rows, CR levels)
 };
 template <typename T> struct SolvePlan {
     int kind = kPlanTile;
@@ -686,7 +696,10 @@ static SolvePlan<T> make_plan(int algo, int layout, int64_t n, int64_t batch, in
     }
     int form = 0;
     bool lds = false;
-    if (!il && bb != 0 && bulk_forms_requested() && bulk_form(algo, form, lds)) {
+    // (the register CR form holds at most 1024 R rows, as the reference's does;
+    // longer systems of those ids keep the default path)
+    if (!il && bb != 0 && bulk_forms_requested() && bulk_form(algo, form, lds) &&
+        (form != 5 || n <= 1024 * (int64_t)forced_M(algo))) {
         pl.kind = kPlanBulk;
         return pl;
     }
@@ -807,7 +820,7 @@ static int solve_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b
     return err == hipSuccess ? TRIDIAG_OK : hip_fail(err, "tile launch");
 }
 
-// The bulk-row form of algorithm `algo` (ids 2-10) in the contiguous or packed
+// The bulk-row form of algorithm `algo` (ids 2-13) in the contiguous or packed
 // layout; batches whose arrays reach 2 GiB run in sub-2-GiB launches.
 template <typename T>
 static int bulk_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b, T *c, T *d,
@@ -816,7 +829,12 @@ static int bulk_impl(int algo, int layout, int64_t n, int64_t batch, T *a, T *b,
     bool lds = false;
     if (!bulk_form(algo, form, lds))
         return fail(TRIDIAG_EUNSUPPORTED, "algorithm " + std::to_string(algo) +
-                                              " has no bulk-row form (ids 2-10 do)");
+                                              " has no bulk-row form (ids 2-13 do)");
+    if (form == 5 && n > 1024 * (int64_t)forced_M(algo))
+        return fail(TRIDIAG_EUNSUPPORTED, "register CR over " + std::to_string(forced_M(algo)) +
+                                              " rows per lane holds at most " +
+                                              std::to_string(1024 * forced_M(algo)) +
+                                              " rows in one workgroup");
     if (n < 1) return fail(TRIDIAG_EINVAL, "n_eqt must be >= 1");
     if (batch < 0) return fail(TRIDIAG_EINVAL, "n_batch must be >= 0");
     if (layout == kInterleaved)

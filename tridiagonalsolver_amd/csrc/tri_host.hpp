@@ -134,6 +134,14 @@ hipError_t launch_bulk_cr_f32(const float *a, const float *b, float *c, float *d
 hipError_t launch_bulk_cr_f64(const double *a, const double *b, double *c, double *d, int64_t n,
                               int64_t batch, int64_t sstride, int32_t bb, double *scratch,
                               hipStream_t st);
+// The reference's register CR ids (CR_reg4/8/16, tri_bulk.hip): R rows per lane,
+// one workgroup per system of at most 1024 R rows.
+hipError_t launch_bulk_crreg_f32(int R, const float *a, const float *b, float *c, float *d,
+                                 int64_t n, int64_t batch, int64_t sstride, int32_t bb,
+                                 hipStream_t st);
+hipError_t launch_bulk_crreg_f64(int R, const double *a, const double *b, double *c, double *d,
+                                 int64_t n, int64_t batch, int64_t sstride, int32_t bb,
+                                 hipStream_t st);
 // Composed adjoint (systems the fused adjoint kernels do not hold): the shifted
 // coefficients of A^T into ap / cp, g into lam, and d into xo when d is given;
 // then, after the solves, the coefficient gradients from x and lambda.

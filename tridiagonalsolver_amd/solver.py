@@ -62,10 +62,11 @@ def solve(a, b, c, d, algo=ALGO_AUTO, layout=CONTIGUOUS, n=None, batch=None):
 
 
 def solve_bulk(a, b, c, d, algo, layout=CONTIGUOUS, n=None, batch=None):
-    """The reference's bulk-row form of algorithm `algo` (id 2: CR through global
+    """The reference's own form of algorithm `algo` (id 2: CR through global
     memory; ids 3-10: PCR-pThomas, PCR-half-pThomas, WM-pGE, WM-half-pGE,
-    shared-memory or register exchanges), in place like solve(): x into d (and,
-    for ids 3-10, c' into c).  Layouts 0 and 1."""
+    shared-memory or register exchanges; ids 11-13: CR over 4, 8, 16 rows per
+    lane, n <= 1024 R), in place like solve(): x into d (and, for ids 3-10, c'
+    into c).  Layouts 0 and 1."""
     ts = [a] if layout == PACKED else [a, b, c, d]
     _check_tensors(ts)
     n, batch = _shape(a if layout == PACKED else d, layout, n, batch)
