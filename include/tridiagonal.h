@@ -158,8 +158,9 @@ int tridiag_solve(int precision, int algo, int layout, int64_t n_eqt, int64_t n_
  * the shared-memory ids (3, 4, 7, 8) exchange rows between lanes through LDS,
  * the register ids (5, 6, 9, 10) through ds_bpermute.  tridiag_solve runs the
  * on-chip partition kernel for these ids unless $TRIDIAG_FORM=bulk; this entry
- * always runs the bulk-row form.  Layouts 0 and 1; any n_eqt >= 1 (the
- * reference needs n % 64 for WM and n % 32 for PCR).  Other ids and the
+ * always runs the reference's form.  Layouts 0 and 1; any n_eqt >= 1 up to the
+ * bound of ids 11-13 (the reference needs n % 64 for WM and n % 32 for PCR).
+ * Other ids and the
  * interleaved layout return TRIDIAG_EUNSUPPORTED.  Asynchronous on `stream`.
  * tridiag_solve_noalloc refuses id 2 under $TRIDIAG_FORM=bulk (its scratch). */
 int tridiag_solve_bulk(int precision, int algo, int layout, int64_t n_eqt, int64_t n_batch,
