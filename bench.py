@@ -339,14 +339,16 @@ def run_pcr(torch, T, steps, warmup, algo=5):
 
 # (id, name, words of memory traffic per unknown by design)
 BULK_IDS = [(2, "CR, levels in global memory", 23), (3, "shmem PCR-pThomas", 9),
-            (5, "reg PCR-pThomas", 9), (7, "shmem WM-pGE", 9), (9, "reg WM-pGE", 9)]
+            (5, "reg PCR-pThomas", 9), (7, "shmem WM-pGE", 9), (9, "reg WM-pGE", 9),
+            (12, "reg-8 CR, 8 rows per lane", 5)]
 
 
 def run_bulk(torch, T, steps, warmup):
     """The reference's own streaming forms of PCR-pThomas and WM-pGE
     (tridiag_solve_bulk: one wave per system, c' and d' through memory, 9 words
-    per unknown) and of CR (its reduced levels written and read back through
-    global memory: at most 23 words per unknown, 11.5 per row of each level) at
+    per unknown), of CR (its reduced levels written and read back through
+    global memory: at most 23 words per unknown, 11.5 per row of each level) and
+    of register CR (8 rows per lane, one workgroup per system: 5 words) at
     n=4096 x 4096 contiguous, FP32 and FP64, beside the partition kernel
     tridiag_solve runs for the same ids."""
     import numpy as np
