@@ -937,7 +937,7 @@ def main():
         "cpu_baseline": cpu,
         "workloads": {k: {kk: v[kk] for kk in ("unknowns_per_s", "ms_per_step", "kernel_ms",
                                                "reference_style_sync_ms", "layout", "plan",
-                                               "parity", "roofline")}
+                                               "array_spacing_mib", "parity", "roofline")}
                       for k, v in res.items()},
         "pcr_4096x4096": pcr,
         "bulk_row_forms_4096x4096": bulk,
