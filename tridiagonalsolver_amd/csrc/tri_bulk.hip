@@ -3,8 +3,9 @@
 // semantics (c' into c and d' into d on the way down, x into d on the way up:
 // 9 words per unknown).  These are the algorithms of the reference's ids 3-10
 // themselves, for callers that want them rather than the partition kernel that
-// tridiag_solve runs for those ids by default; id 2, cyclic reduction with its
-// levels in global memory, is at the end.
+// tridiag_solve runs for those ids by default.  At the end: id 2, cyclic
+// reduction with its levels in global memory (one wave per system), and ids
+// 11-13, cyclic reduction over R rows per lane (one workgroup per system).
 //
 //   WM-pGE (Tridiagonal_CUDA/tridiagonal_wm_pcr.cu:17-205, 424-646; WM-half-pGE
 //   :207-422, 648-884).  A chunk is 128 rows: lane t holds rows t and 64 + t.
