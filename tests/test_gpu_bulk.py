@@ -80,7 +80,7 @@ def test_bulk_low_margin(prec, algo):
     assert st["nonfinite"] == 0 and st["relerr"] <= TOL[DT[prec]], st
 
 
-@pytest.mark.parametrize("algo", [2, 3, 9])
+@pytest.mark.parametrize("algo", [2, 3, 9, 12])
 def test_bulk_packed_layout(algo):
     n, batch = 1000, 33
     announce("bulk packed", algo)
