@@ -347,7 +347,7 @@ def run_bulk(torch, T, steps, warmup):
     """The reference's own streaming forms of PCR-pThomas and WM-pGE
     (tridiag_solve_bulk: one wave per system, c' and d' through memory, 9 words
     per unknown), of CR (its reduced levels written and read back through
-    global memory: at most 23 words per unknown, 11.5 per row of each level) and
+    global memory: about 23 words per unknown, 11.5 per row of each level) and
     of register CR (8 rows per lane, one workgroup per system: 5 words) at
     n=4096 x 4096 contiguous, FP32 and FP64, beside the partition kernel
     tridiag_solve runs for the same ids."""
